@@ -1,0 +1,280 @@
+"""libflame_amd -- MI355X (gfx950) implementation of libflame's Cholesky / LU hot path.
+
+Python host mirror of the C API in include/FLAME.h and include/FLA_hip.h,
+through ctypes over the in-tree ``libflame_amd/libflame_amd.so``.  Function
+names, argument meaning, return values and error behaviour follow libflame:
+``FLA_Chol(uplo, A)`` returns FLA_SUCCESS (-1) or the 0-based index of the
+first failing column, ``FLA_LU_piv(A, p)`` the 0-based index of the first
+null pivot, and pivots are FLAME-native (0-based, relative to the row).
+
+The library must be built (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C libflame_amd``); importing this package without it raises.
+There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflame_amd.so")
+
+# ---- constants (FLA_macro_defs.h values) ------------------------------------
+FLA_SUCCESS, FLA_FAILURE = -1, -2
+FLA_TL, FLA_TR, FLA_BL, FLA_BR = 11, 12, 21, 22
+FLA_FLOAT, FLA_DOUBLE, FLA_COMPLEX, FLA_DOUBLE_COMPLEX, FLA_INT, FLA_CONSTANT = range(100, 106)
+FLA_TOP, FLA_BOTTOM, FLA_LEFT, FLA_RIGHT = 200, 201, 210, 211
+FLA_LOWER_TRIANGULAR, FLA_UPPER_TRIANGULAR = 300, 301
+FLA_ZERO_MATRIX, FLA_FULL_MATRIX = 310, 311
+FLA_NO_TRANSPOSE, FLA_TRANSPOSE, FLA_CONJ_TRANSPOSE = 400, 401, 402
+FLA_UNIT_DIAG, FLA_NONUNIT_DIAG = 500, 501
+FLA_NATIVE_PIVOTS, FLA_LAPACK_PIVOTS = 700, 701
+FLA_NO_ERROR_CHECKING, FLA_MIN_ERROR_CHECKING, FLA_FULL_ERROR_CHECKING = 0, 1, 2
+FLA_HIP_OK = 0
+
+
+class FLA_Base_obj(C.Structure):
+    _fields_ = [("datatype", C.c_int), ("elemtype", C.c_int), ("m", C.c_ulong), ("n", C.c_ulong),
+                ("rs", C.c_ulong), ("cs", C.c_ulong), ("m_inner", C.c_ulong),
+                ("n_inner", C.c_ulong), ("id", C.c_ulong), ("m_index", C.c_ulong),
+                ("n_index", C.c_ulong), ("n_elem_alloc", C.c_ulong), ("buffer", C.c_void_p),
+                ("buffer_info", C.c_int), ("uplo", C.c_int)]
+
+
+class FLA_Obj(C.Structure):
+    _fields_ = [("offm", C.c_ulong), ("offn", C.c_ulong), ("m", C.c_ulong), ("n", C.c_ulong),
+                ("m_inner", C.c_ulong), ("n_inner", C.c_ulong),
+                ("base", C.POINTER(FLA_Base_obj))]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                               "(libflame_amd has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        O = FLA_Obj
+        OP = C.POINTER(FLA_Obj)
+        sig = {
+            "FLA_Init": (None, []), "FLA_Finalize": (None, []),
+            "FLA_Initialized": (C.c_int, []),
+            "FLA_Check_error_level": (C.c_uint, []),
+            "FLA_Check_error_level_set": (C.c_uint, [C.c_uint]),
+            "FLA_Obj_create": (C.c_int, [C.c_int, C.c_ulong, C.c_ulong, C.c_ulong, C.c_ulong, OP]),
+            "FLA_Obj_create_without_buffer": (C.c_int, [C.c_int, C.c_ulong, C.c_ulong, OP]),
+            "FLA_Obj_attach_buffer": (C.c_int, [vp, C.c_ulong, C.c_ulong, OP]),
+            "FLA_Obj_free": (C.c_int, [OP]), "FLA_Obj_free_without_buffer": (C.c_int, [OP]),
+            "FLA_Obj_buffer_at_view": (vp, [O]),
+            "FLA_Obj_is_device": (C.c_int, [O]),
+            "FLA_Part_2x2": (C.c_int, [O, OP, OP, OP, OP, C.c_ulong, C.c_ulong, C.c_int]),
+            "FLA_Chol": (C.c_int, [C.c_int, O]),
+            "FLA_LU_piv": (C.c_int, [O, O]),
+            "FLA_Apply_pivots": (C.c_int, [C.c_int, C.c_int, O, O]),
+            "FLA_Chol_solve": (C.c_int, [C.c_int, O, O, O]),
+            "FLA_LU_piv_solve": (C.c_int, [O, O, O, O]),
+            "FLA_Shift_pivots_to": (C.c_int, [C.c_int, O]),
+            "dpotrf_": (C.c_int, [C.c_char_p, ip, dp, ip, ip]),
+            "dgetrf_": (C.c_int, [ip, ip, dp, ip, ip, ip]),
+            "dpotrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, ip]),
+            "dgetrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, ip, dp, ip, ip]),
+            "fla_hip_init": (C.c_int, []), "fla_hip_finalize": (C.c_int, []),
+            "fla_hip_synchronize": (C.c_int, []),
+            "fla_hip_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+            "fla_hip_free": (C.c_int, [vp]),
+            "fla_hip_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
+            "fla_hip_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+            "fla_hip_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
+            "fla_hip_error_string": (C.c_char_p, [C.c_int]),
+            "fla_hip_dgemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        vp, C.c_int, vp, C.c_int, C.c_double, vp, C.c_int]),
+            "fla_hip_dsyrk": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp,
+                                        C.c_int, C.c_double, vp, C.c_int]),
+            "fla_hip_dtrsm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, vp, C.c_int, vp, C.c_int]),
+            "fla_hip_dpotrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
+            "fla_hip_dgetrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, ip]),
+            "fla_hip_dlaswp": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_shift_pivots": (C.c_int, [C.c_int, C.c_int, vp]),
+            "fla_hip_stats_enable": (C.c_int, [C.c_int]),
+            "fla_hip_stats_reset": (C.c_int, []),
+            "fla_hip_stats_get": (C.c_int, [C.c_int, dp, dp, C.POINTER(C.c_long)]),
+            "fla_hip_set_blocksizes": (C.c_int, [C.c_int, C.c_int]),
+            "fla_hip_get_blocksizes": (C.c_int, [ip, ip]),
+            "fla_hip_gen_uniform": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_gen_normal": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_gram_lower": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_double,
+                                             C.c_double, vp, C.c_int]),
+            "fla_hip_scale_rows_logspace": (C.c_int, [C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_dmatmul_check": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.c_int, vp, C.c_int, vp, C.c_int, C.c_double,
+                                                vp, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what="libflame_amd call"):
+    if status != FLA_HIP_OK:
+        msg = lib().fla_hip_error_string(status).decode()
+        raise RuntimeError(f"{what} failed: {msg} ({status})")
+
+
+class DeviceBuffer:
+    """Owned device allocation (fla_hip_malloc / fla_hip_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(lib().fla_hip_malloc(C.byref(self.ptr), max(self.nbytes, 8)), "fla_hip_malloc")
+
+    def free(self):
+        if self.ptr:
+            lib().fla_hip_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+
+class DeviceMatrix:
+    """Column-major m x n device matrix of float64 (or int32 vectors)."""
+
+    def __init__(self, m, n, dtype=np.float64, ld=None):
+        self.m, self.n = int(m), int(n)
+        self.dtype = np.dtype(dtype)
+        self.ld = int(ld) if ld is not None else max(1, self.m)
+        self.buf = DeviceBuffer(self.ld * max(self.n, 1) * self.dtype.itemsize)
+
+    @property
+    def ptr(self):
+        return self.buf.ptr
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.asarray(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        dtype = np.dtype(dtype or a.dtype)
+        out = cls(a.shape[0], a.shape[1], dtype)
+        out.upload(a)
+        return out
+
+    def upload(self, a):
+        a = np.asfortranarray(np.asarray(a, dtype=self.dtype).reshape(self.m, self.n))
+        if self.ld != self.m:
+            raise ValueError("upload expects a packed matrix")
+        check(lib().fla_hip_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes), "upload")
+
+    def download(self):
+        out = np.empty((self.ld, self.n), dtype=self.dtype, order="F")
+        if out.nbytes:
+            check(lib().fla_hip_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "download")
+        return out[: self.m, :]
+
+    def obj(self, datatype=None):
+        """An FLA_Obj with this device buffer attached (the Python object keeps
+        the base alive)."""
+        dt = datatype or (FLA_INT if self.dtype == np.int32 else FLA_DOUBLE)
+        return Obj.attach(self.ptr.value, self.m, self.n, dt, self.ld)
+
+
+class Obj:
+    """FLA_Obj created without buffer and attached to a given address."""
+
+    def __init__(self):
+        self.o = FLA_Obj()
+
+    @classmethod
+    def attach(cls, addr, m, n, datatype, ld):
+        self = cls()
+        lib().FLA_Obj_create_without_buffer(datatype, m, n, C.byref(self.o))
+        lib().FLA_Obj_attach_buffer(C.c_void_p(addr), 1, max(1, ld), C.byref(self.o))
+        return self
+
+    def __del__(self):
+        try:
+            if self.o.base:
+                lib().FLA_Obj_free_without_buffer(C.byref(self.o))
+        except Exception:
+            pass
+
+
+def FLA_Init():
+    lib().FLA_Init()
+
+
+def FLA_Finalize():
+    lib().FLA_Finalize()
+
+
+def FLA_Chol(uplo, A):
+    return lib().FLA_Chol(uplo, A.o if isinstance(A, Obj) else A)
+
+
+def FLA_LU_piv(A, p):
+    return lib().FLA_LU_piv(A.o if isinstance(A, Obj) else A, p.o if isinstance(p, Obj) else p)
+
+
+def FLA_Apply_pivots(side, trans, p, A):
+    return lib().FLA_Apply_pivots(side, trans, p.o, A.o)
+
+
+def FLA_Chol_solve(uplo, A, B, X):
+    return lib().FLA_Chol_solve(uplo, A.o, B.o, X.o)
+
+
+def FLA_LU_piv_solve(A, p, B, X):
+    return lib().FLA_LU_piv_solve(A.o, p.o, B.o, X.o)
+
+
+def FLA_Shift_pivots_to(ptype, p):
+    return lib().FLA_Shift_pivots_to(ptype, p.o)
+
+
+def synchronize():
+    check(lib().fla_hip_synchronize(), "synchronize")
+
+
+def set_blocksizes(chol_nb=0, lu_nb=0):
+    check(lib().fla_hip_set_blocksizes(chol_nb, lu_nb), "set_blocksizes")
+
+
+STATS_CHOL_UPDATE, STATS_LU_UPDATE, STATS_LU_PANEL, STATS_CHOL_PANEL = 0, 1, 2, 3
+
+
+def stats_enable(on=True):
+    lib().fla_hip_stats_enable(1 if on else 0)
+
+
+def stats_reset():
+    lib().fla_hip_stats_reset()
+
+
+def stats_get(kind):
+    """(algorithmic flops, milliseconds, launches) accumulated for `kind`."""
+    f, ms, n = C.c_double(), C.c_double(), C.c_long()
+    check(lib().fla_hip_stats_get(kind, C.byref(f), C.byref(ms), C.byref(n)), "stats_get")
+    return f.value, ms.value, n.value
+
+
+def get_blocksizes():
+    a, b = C.c_int(), C.c_int()
+    lib().fla_hip_get_blocksizes(C.byref(a), C.byref(b))
+    return a.value, b.value

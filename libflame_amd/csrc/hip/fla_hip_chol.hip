@@ -1,0 +1,255 @@
+// Blocked Cholesky on one MI355X: diagonal-block factor, panel TRSM and the
+// MFMA trailing update, scheduled with one panel of look-ahead on two streams.
+//
+// Reference algorithm: FLA_Chol_l_blk_var3
+//   (src/lapack/dec/chol/l/flamec/FLA_Chol_l_blk_var3.c:13-66):
+//     A11 := chol(A11); A21 := A21 * inv(tril(A11))'; A22 := A22 - A21*A21'
+// and its leaf FLA_Chol_l_opd_var2 (src/lapack/dec/chol/l/flamec/FLA_Chol_l_opt_var2.c:136-193),
+// whose failure rule (pivot <= 0 or NaN, bl1_dsqrte in
+// src/base/flamec/include/blis_macro_defs.h:134-136) and 0-based failing
+// column index (FLA_Chol_l_blk_var3.c:42-43) are kept.  The upper case
+// (FLA_Chol_u_blk_var3.c:38-54) is the lower algorithm on A' (the triangles
+// are exchanged in place before and after).
+#include "fla_hip_internal.h"
+
+namespace {
+
+constexpr int PB = 128;         // panel sub-block width (diagonal block order)
+constexpr int PLD = PB + 1;     // LDS leading dimension of the diagonal block
+constexpr int POTRF_THREADS = 1024;
+
+// Single-workgroup Cholesky of the nb x nb lower block at A (nb <= 128), kept
+// in LDS.  Right-looking, one column per step: the pivot check and sqrt are
+// computed redundantly by every thread (identical values, no extra barrier),
+// the scaled column goes through a separate LDS vector so that the trailing
+// update reads unscaled-free values, and two barriers separate the steps.
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(int nb, double* __restrict__ A,
+                                                                    long lda, int col_offset,
+                                                                    int* __restrict__ info) {
+  if (*info >= 0) return;
+  __shared__ double S[PB * PLD];
+  __shared__ double lcol[PB];
+  __shared__ int fail;
+  const int t = threadIdx.x;
+  if (t == 0) fail = -1;
+  for (int idx = t; idx < nb * nb; idx += POTRF_THREADS) {
+    const int i = idx % nb, c = idx / nb;
+    S[c * PLD + i] = (i >= c) ? A[(long)i + (long)c * lda] : 0.0;
+  }
+  __syncthreads();
+  const int i = t & (PB - 1);
+  const int c0 = t >> 7;  // 0..7
+  int j = 0;
+  for (; j < nb; ++j) {
+    const double d2 = S[j * PLD + j];
+    if (!(d2 > 0.0)) {  // <= 0 or NaN, as bl1_dsqrte
+      if (t == 0) fail = j;
+      break;
+    }
+    const double d = sqrt(d2);
+    const double inv = 1.0 / d;
+    if (t < nb && t > j) lcol[t] = S[j * PLD + t] * inv;
+    __syncthreads();
+    if (t == 0) S[j * PLD + j] = d;
+    if (t < nb && t > j) S[j * PLD + t] = lcol[t];
+    if (i < nb && i > j) {
+      const double li = lcol[i];
+#pragma unroll 4
+      for (int s = 0; s < PB / 8; ++s) {
+        const int c = c0 + 8 * s;
+        if (c > j && c <= i) S[c * PLD + i] -= li * lcol[c];
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  const int ncols = (fail >= 0) ? fail : nb;
+  for (int idx = t; idx < nb * nb; idx += POTRF_THREADS) {
+    const int ii = idx % nb, c = idx / nb;
+    if (ii >= c && c < ncols) A[(long)ii + (long)c * lda] = S[c * PLD + ii];
+  }
+  if (t == 0 && fail >= 0) *info = col_offset + fail;
+}
+
+// A21 := A21 * inv(L11)' for an m x 128 strip: one lane per row, the row held
+// in 128 VGPR pairs, L11 read through uniform (scalar) loads.  Same update
+// order as dtrsm_('R','L','T','N'): x_j *= 1/l_jj, then x_c -= l_cj * x_j.
+__global__ __launch_bounds__(64) void trsm_rlt_kernel(int m, const double* __restrict__ L,
+                                                      long ldl, double* __restrict__ A21,
+                                                      long lda, const int* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag >= 0) return;
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  const bool active = r < m;
+  double x[PB];
+#pragma unroll
+  for (int c = 0; c < PB; ++c) x[c] = active ? A21[(long)r + (long)c * lda] : 0.0;
+#pragma unroll
+  for (int j = 0; j < PB; ++j) {
+    const double inv = 1.0 / L[(long)j + (long)j * ldl];
+    x[j] *= inv;
+#pragma unroll
+    for (int c = j + 1; c < PB; ++c) x[c] -= L[(long)c + (long)j * ldl] * x[j];
+  }
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < PB; ++c) A21[(long)r + (long)c * lda] = x[c];
+  }
+}
+
+// Exchanges A(i,j) and A(j,i) for all i > j, 32x32 tile pairs through LDS.
+__global__ __launch_bounds__(256) void transpose_swap_kernel(int n, double* __restrict__ A, long lda,
+                                                             int T) {
+  __shared__ double P[32][33];
+  __shared__ double Q[32][33];
+  // Lower tile (I, J), I >= J, enumerated row by row.
+  const int b = blockIdx.x;
+  int I = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+  while ((long)I * (I + 1) / 2 > b) --I;
+  while ((long)(I + 1) * (I + 2) / 2 <= b) ++I;
+  const int J = b - I * (I + 1) / 2;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty 0..7
+  const int ri = I * 32, cj = J * 32;
+  for (int y = ty; y < 32; y += 8) {
+    const int i = ri + tx, j = cj + y;  // lower tile element (i, j)
+    P[y][tx] = (i < n && j < n) ? A[(long)i + (long)j * lda] : 0.0;
+    const int i2 = cj + tx, j2 = ri + y;  // upper tile element (i2, j2)
+    Q[y][tx] = (i2 < n && j2 < n) ? A[(long)i2 + (long)j2 * lda] : 0.0;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    // new lower (ri+tx, cj+y) := old upper (cj+y, ri+tx) = Q[ri+tx - ri][cj+y - cj]
+    const int i = ri + tx, j = cj + y;
+    if (i < n && j < n && i > j) A[(long)i + (long)j * lda] = Q[tx][y];
+    const int i2 = cj + tx, j2 = ri + y;
+    if (i2 < n && j2 < n && i2 < j2) A[(long)i2 + (long)j2 * lda] = P[tx][y];
+  }
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+}  // namespace
+
+int launch_potrf_diag(hipStream_t s, int nb, double* A, long lda, int col_offset, int* info) {
+  if (nb <= 0) return FLA_HIP_OK;
+  if (nb > PB) return FLA_HIP_EARG;
+  hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(POTRF_THREADS), 0, s, nb, A, lda,
+                     col_offset, info);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+int launch_trsm_rlt(hipStream_t s, int m, int nb, const double* L11, long ldl, double* A21,
+                    long lda, const int* abort_flag) {
+  if (m <= 0) return FLA_HIP_OK;
+  if (nb != PB) return FLA_HIP_EARG;
+  hipLaunchKernelGGL(trsm_rlt_kernel, dim3((m + 63) / 64), dim3(64), 0, s, m, L11, ldl, A21,
+                     lda, abort_flag);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+int launch_transpose_swap(hipStream_t s, int n, double* A, long lda) {
+  if (n <= 1) return FLA_HIP_OK;
+  const int T = (n + 31) / 32;
+  const long nblk = (long)T * (T + 1) / 2;
+  hipLaunchKernelGGL(transpose_swap_kernel, dim3((unsigned)nblk), dim3(256), 0, s, n, A, lda, T);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+// Factors columns [k, k+kb) of the lower matrix (rows k..n-1): sub-blocks of
+// 128 = diagonal factor, strip TRSM of everything below, and the update of the
+// remaining panel columns.  All on stream s.
+static int chol_panel(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info) {
+  for (int kk = k; kk < k + kb; kk += PB) {
+    const int ib = (k + kb - kk) < PB ? (k + kb - kk) : PB;
+    double* Akk = A + (long)kk + (long)kk * lda;
+    FLA_TRY(launch_potrf_diag(s, ib, Akk, lda, kk, info));
+    const int mb = n - kk - ib;
+    if (mb <= 0) continue;
+    FLA_TRY(launch_trsm_rlt(s, mb, ib, Akk, lda, Akk + ib, lda, info));
+    const int w = k + kb - kk - ib;  // remaining columns of this panel
+    if (w > 0) {
+      const double* P = Akk + ib;  // mb x ib
+      FLA_TRY(launch_dgemm_mfma(s, false, true, 1, mb, w, ib, -1.0, P, lda, P, lda, 1.0,
+                                A + (long)(kk + ib) + (long)(kk + ib) * lda, lda, info));
+    }
+  }
+  return FLA_HIP_OK;
+}
+
+// Lower blocked Cholesky with look-ahead: the panel of step k+1 runs on
+// s_panel while the bulk of step k's trailing update runs on s_update.
+static int chol_lower(int n, double* A, long lda, int* dinfo) {
+  FlaHipCtx& c = fla_ctx();
+  const int nb = c.chol_nb;
+  hipStream_t sp = c.s_panel, su = c.s_update;
+  int evi = 0;
+  auto next_event = [&]() { hipEvent_t e = c.ev[evi]; evi = (evi + 1) % FlaHipCtx::kEvents; return e; };
+
+  // Both streams start after whatever the caller queued on s_update.
+  hipEvent_t start = next_event();
+  FLA_HIP_CHECK(hipEventRecord(start, su));
+  FLA_HIP_CHECK(hipStreamWaitEvent(sp, start, 0));
+
+  // Panel 0.
+  int kb0 = n < nb ? n : nb;
+  FLA_TRY(chol_panel(sp, n, A, lda, 0, kb0, dinfo));
+  for (int k = 0; k < n;) {
+    const int kb = (n - k) < nb ? (n - k) : nb;
+    hipEvent_t panel_done = next_event();
+    FLA_HIP_CHECK(hipEventRecord(panel_done, sp));
+    FLA_HIP_CHECK(hipStreamWaitEvent(su, panel_done, 0));
+    const int k2 = k + kb;      // first column of the trailing matrix
+    const int m2 = n - k2;      // its order
+    if (m2 <= 0) break;
+    const double* P = A + (long)k2 + (long)k * lda;  // m2 x kb, the factored panel
+    const int kb2 = m2 < nb ? m2 : nb;             // width of the next panel
+    // (a) next panel's columns: rows k2.., columns [k2, k2+kb2) (lower part)
+    void* tok = fla_stats_begin(su);
+    FLA_TRY(launch_dgemm_mfma(su, false, true, 1, m2, kb2, kb, -1.0, P, lda, P, lda, 1.0,
+                              A + (long)k2 + (long)k2 * lda, lda, dinfo));
+    fla_stats_end(su, tok, 0, lower_update_flops(m2, kb2, kb));
+    hipEvent_t next_ready = next_event();
+    FLA_HIP_CHECK(hipEventRecord(next_ready, su));
+    // (b) the rest of the trailing matrix
+    const int m3 = m2 - kb2;
+    if (m3 > 0) {
+      const double* P3 = P + kb2;
+      tok = fla_stats_begin(su);
+      FLA_TRY(launch_dgemm_mfma(su, false, true, 1, m3, m3, kb, -1.0, P3, lda, P3, lda, 1.0,
+                                A + (long)(k2 + kb2) + (long)(k2 + kb2) * lda, lda, dinfo));
+      fla_stats_end(su, tok, 0, lower_update_flops(m3, m3, kb));
+    }
+    // Next panel on the panel stream, concurrent with (b).
+    FLA_HIP_CHECK(hipStreamWaitEvent(sp, next_ready, 0));
+    FLA_TRY(chol_panel(sp, n, A, lda, k2, kb2, dinfo));
+    k = k2;
+  }
+  // Join: s_update waits for the last panel.
+  hipEvent_t fin = next_event();
+  FLA_HIP_CHECK(hipEventRecord(fin, sp));
+  FLA_HIP_CHECK(hipStreamWaitEvent(su, fin, 0));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dpotrf(int uplo, int n, double* A, int lda, int* info) {
+  FLA_TRY(fla_ensure_ready());
+  if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
+  if (n < 0 || lda < (n > 1 ? n : 1)) return FLA_HIP_EARG;
+  *info = kSuccess;
+  if (n == 0) return FLA_HIP_OK;
+  FlaHipCtx& c = fla_ctx();
+  int* dinfo = c.dscratch;
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, c.s_update, dinfo, kSuccess);
+  FLA_HIP_CHECK(hipGetLastError());
+  if (uplo == kUpper) FLA_TRY(launch_transpose_swap(c.s_update, n, A, lda));
+  FLA_TRY(chol_lower(n, A, lda, dinfo));
+  if (uplo == kUpper) FLA_TRY(launch_transpose_swap(c.s_update, n, A, lda));
+  int h = kSuccess;
+  FLA_HIP_CHECK(hipMemcpyAsync(&h, dinfo, sizeof(int), hipMemcpyDeviceToHost, c.s_update));
+  FLA_HIP_CHECK(hipStreamSynchronize(c.s_update));
+  fla_stats_harvest();
+  *info = h;
+  return FLA_HIP_OK;
+}

@@ -1,0 +1,93 @@
+// Internal declarations shared by the HIP translation units of libflame_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+
+#include "FLA_hip.h"
+
+// FLAME constants (same numeric values as the reference's FLA_macro_defs.h).
+enum : int {
+  kSuccess = -1,
+  kLeft = 210,
+  kRight = 211,
+  kLower = 300,
+  kUpper = 301,
+  kFull = 311,
+  kNoTrans = 400,
+  kTrans = 401,
+  kConjTrans = 402,
+  kUnit = 500,
+  kNonUnit = 501,
+  kNativePivots = 700,
+  kLapackPivots = 701,
+};
+
+struct FlaHipCtx {
+  bool ready = false;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t s_update = nullptr;  // trailing updates (and the API's default stream)
+  hipStream_t s_panel = nullptr;   // panel factorizations (look-ahead)
+  static constexpr int kEvents = 8;
+  hipEvent_t ev[kEvents] = {};
+  // Grow-only device workspace for scratch (info words, counters, panels).
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  // Small persistent device scratch: [0] info word, [1..] counters.
+  int* dscratch = nullptr;
+  int chol_nb = 256;
+  int lu_nb = 128;
+};
+
+FlaHipCtx& fla_ctx();
+int fla_ensure_ready();
+// Returns a device workspace of at least `bytes` bytes (grow-only).
+int fla_workspace(size_t bytes, void** ptr);
+
+#define FLA_HIP_CHECK(expr)                                                    \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      std::fprintf(stderr, "libflame_amd: %s failed at %s:%d: %s\n", #expr,    \
+                   __FILE__, __LINE__, hipGetErrorString(e_));                 \
+      return (int)e_;                                                          \
+    }                                                                          \
+  } while (0)
+
+#define FLA_TRY(expr)              \
+  do {                             \
+    int s_ = (expr);               \
+    if (s_ != FLA_HIP_OK) return s_; \
+  } while (0)
+
+// ---- measurement hooks (no-ops unless fla_hip_stats_enable(1)) -------------
+bool fla_stats_on();
+void* fla_stats_begin(hipStream_t s);
+void fla_stats_end(hipStream_t s, void* token, int kind, double flops);
+void fla_stats_harvest();
+// Algorithmic flops of a lower update of an m x n block (top n x n part on
+// the diagonal) with inner dimension k: 2k per stored element.
+inline double lower_update_flops(int m, int n, int k) {
+  const double nn = n, mm = m;
+  return 2.0 * k * ((mm - nn) * nn + nn * (nn + 1.0) / 2.0);
+}
+
+// ---- kernel launchers implemented in the other translation units ----------
+// MFMA FP64 update: C := alpha*op(A)*op(B) + beta*C (mode 0 full, 1 lower).
+int launch_dgemm_mfma(hipStream_t s, bool ta, bool tb, int mode, int m, int n, int k,
+                      double alpha, const double* A, long lda, const double* B, long ldb,
+                      double beta, double* C, long ldc, const int* abort_flag);
+// Single-workgroup Cholesky of an nb x nb lower diagonal block (nb <= 128).
+int launch_potrf_diag(hipStream_t s, int nb, double* A, long lda, int col_offset,
+                      int* info);
+// A21 := A21 * inv(L11)' for an m x nb strip, L11 lower non-unit (nb <= 128).
+int launch_trsm_rlt(hipStream_t s, int m, int nb, const double* L11, long ldl, double* A21,
+                    long lda, const int* abort_flag);
+// A12 := inv(L11) * A12, L11 unit lower 128 x 128: one lane per column.
+int launch_trsm_llnu(hipStream_t s, int ncols, const double* L11, long ldl, double* A12,
+                     long lda, const int* abort_flag);
+// In-place exchange of the strictly lower and strictly upper triangles.
+int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

@@ -1,0 +1,280 @@
+// Runtime of libflame_amd: device selection, the two HIP streams that replace
+// SuperMatrix's worker threads, a grow-only workspace, and host<->device copies.
+//
+// The reference binds one rocBLAS handle per SuperMatrix thread and device
+// (FLASH_Queue_bind_hip, src/base/flamec/supermatrix/hip/main/FLASH_Queue_hip.c:297-318);
+// here one process drives one GPU through two streams: s_update for trailing
+// updates and s_panel for the look-ahead panel chain.
+#include <cstring>
+
+#include "fla_hip_internal.h"
+
+static FlaHipCtx g_ctx;
+
+FlaHipCtx& fla_ctx() { return g_ctx; }
+
+int fla_ensure_ready() {
+  if (g_ctx.ready) return FLA_HIP_OK;
+  return fla_hip_init();
+}
+
+extern "C" int fla_hip_init(void) {
+  if (g_ctx.ready) return FLA_HIP_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    std::fprintf(stderr, "libflame_amd: no HIP device available\n");
+    return FLA_HIP_ENODEV;
+  }
+  FLA_HIP_CHECK(hipSetDevice(g_ctx.device));
+  hipDeviceProp_t prop;
+  FLA_HIP_CHECK(hipGetDeviceProperties(&prop, g_ctx.device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    std::fprintf(stderr, "libflame_amd: device 0 is %s, this build targets gfx950\n",
+                 prop.gcnArchName);
+    return FLA_HIP_ENODEV;
+  }
+  g_ctx.num_cus = prop.multiProcessorCount;
+  // Default-priority streams only.
+  FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+  FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_panel, hipStreamNonBlocking));
+  for (int i = 0; i < FlaHipCtx::kEvents; ++i)
+    FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
+  FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dscratch), 4096));
+  g_ctx.ready = true;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_finalize(void) {
+  if (!g_ctx.ready) return FLA_HIP_OK;
+  FLA_HIP_CHECK(hipDeviceSynchronize());
+  if (g_ctx.work) FLA_HIP_CHECK(hipFree(g_ctx.work));
+  g_ctx.work = nullptr;
+  g_ctx.work_bytes = 0;
+  if (g_ctx.dscratch) FLA_HIP_CHECK(hipFree(g_ctx.dscratch));
+  g_ctx.dscratch = nullptr;
+  for (int i = 0; i < FlaHipCtx::kEvents; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev[i]));
+  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
+  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_panel));
+  g_ctx.ready = false;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_synchronize(void) {
+  FLA_TRY(fla_ensure_ready());
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_panel));
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
+  return FLA_HIP_OK;
+}
+
+int fla_workspace(size_t bytes, void** ptr) {
+  FLA_TRY(fla_ensure_ready());
+  if (bytes > g_ctx.work_bytes) {
+    // Every queued kernel may still use the old buffer.
+    FLA_HIP_CHECK(hipDeviceSynchronize());
+    if (g_ctx.work) FLA_HIP_CHECK(hipFree(g_ctx.work));
+    g_ctx.work = nullptr;
+    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+    FLA_HIP_CHECK(hipMalloc(&g_ctx.work, want));
+    g_ctx.work_bytes = want;
+  }
+  *ptr = g_ctx.work;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_malloc(void** ptr, size_t bytes) {
+  FLA_TRY(fla_ensure_ready());
+  *ptr = nullptr;
+  if (bytes == 0) return FLA_HIP_OK;
+  FLA_HIP_CHECK(hipMalloc(ptr, bytes));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_free(void* ptr) {
+  if (!ptr) return FLA_HIP_OK;
+  FLA_TRY(fla_ensure_ready());
+  FLA_HIP_CHECK(hipDeviceSynchronize());
+  FLA_HIP_CHECK(hipFree(ptr));
+  return FLA_HIP_OK;
+}
+
+static int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  FLA_TRY(fla_ensure_ready());
+  if (bytes == 0) return FLA_HIP_OK;
+  FLA_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, g_ctx.s_update));
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return copy_sync(dst, src, bytes, hipMemcpyHostToDevice);
+}
+extern "C" int fla_hip_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+extern "C" int fla_hip_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+static int copy2d(int m, int n, const double* s, int lds, double* d, int ldd,
+                  hipMemcpyKind kind) {
+  FLA_TRY(fla_ensure_ready());
+  if (m <= 0 || n <= 0) return FLA_HIP_OK;
+  if (lds < m || ldd < m) return FLA_HIP_EARG;
+  FLA_HIP_CHECK(hipMemcpy2DAsync(d, (size_t)ldd * sizeof(double), s,
+                                 (size_t)lds * sizeof(double), (size_t)m * sizeof(double),
+                                 (size_t)n, kind, g_ctx.s_update));
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dmatrix_h2d(int m, int n, const double* h, int ldh, double* d,
+                                   int ldd) {
+  return copy2d(m, n, h, ldh, d, ldd, hipMemcpyHostToDevice);
+}
+extern "C" int fla_hip_dmatrix_d2h(int m, int n, const double* d, int ldd, double* h,
+                                   int ldh) {
+  return copy2d(m, n, d, ldd, h, ldh, hipMemcpyDeviceToHost);
+}
+extern "C" int fla_hip_dmatrix_d2d(int m, int n, const double* s, int lds, double* d,
+                                   int ldd) {
+  return copy2d(m, n, s, lds, d, ldd, hipMemcpyDeviceToDevice);
+}
+
+extern "C" int fla_hip_memcpy2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                                size_t width, size_t height, int kind) {
+  FLA_TRY(fla_ensure_ready());
+  if (width == 0 || height == 0) return FLA_HIP_OK;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+  FLA_HIP_CHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, k, g_ctx.s_update));
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_is_device_ptr(const void* ptr) {
+  if (!ptr) return 0;
+  if (fla_ensure_ready() != FLA_HIP_OK) return 0;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory: clear the sticky error
+    return 0;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+extern "C" const char* fla_hip_error_string(int status) {
+  switch (status) {
+    case FLA_HIP_OK: return "success";
+    case FLA_HIP_EARG: return "invalid argument";
+    case FLA_HIP_ENODEV: return "no gfx950 device";
+    case FLA_HIP_ESPIN: return "inter-workgroup wait expired";
+    default: return hipGetErrorString((hipError_t)status);
+  }
+}
+
+extern "C" int fla_hip_set_blocksizes(int chol_nb, int lu_nb) {
+  if (chol_nb < 0 || lu_nb < 0) return FLA_HIP_EARG;
+  if (chol_nb > 0) {
+    if (chol_nb % 128 != 0 || chol_nb > 1024) return FLA_HIP_EARG;
+    g_ctx.chol_nb = chol_nb;
+  }
+  if (lu_nb > 0) {
+    if (lu_nb % 32 != 0 || lu_nb > 128) return FLA_HIP_EARG;
+    g_ctx.lu_nb = lu_nb;
+  }
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb) {
+  if (chol_nb) *chol_nb = g_ctx.chol_nb;
+  if (lu_nb) *lu_nb = g_ctx.lu_nb;
+  return FLA_HIP_OK;
+}
+
+// ---- kernel timing for measurement (HIP events on the launching stream) ----
+#include <vector>
+
+namespace {
+struct PendingTiming {
+  hipEvent_t start, stop;
+  int kind;
+  double flops;
+};
+struct StatsState {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<PendingTiming> pending;
+  double flops[4] = {0, 0, 0, 0};
+  double ms[4] = {0, 0, 0, 0};
+  long launches[4] = {0, 0, 0, 0};
+};
+StatsState g_stats;
+
+hipEvent_t take_event() {
+  if (!g_stats.pool.empty()) {
+    hipEvent_t e = g_stats.pool.back();
+    g_stats.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+bool fla_stats_on() { return g_stats.on; }
+
+void* fla_stats_begin(hipStream_t s) {
+  if (!g_stats.on) return nullptr;
+  hipEvent_t e = take_event();
+  if (e) (void)hipEventRecord(e, s);
+  return e;
+}
+
+void fla_stats_end(hipStream_t s, void* token, int kind, double flops) {
+  if (!g_stats.on || !token) return;
+  hipEvent_t e = take_event();
+  if (!e) return;
+  (void)hipEventRecord(e, s);
+  g_stats.pending.push_back({static_cast<hipEvent_t>(token), e, kind, flops});
+}
+
+void fla_stats_harvest() {
+  for (auto& p : g_stats.pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.stop) == hipSuccess &&
+        hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess && p.kind >= 0 && p.kind < 4) {
+      g_stats.ms[p.kind] += ms;
+      g_stats.flops[p.kind] += p.flops;
+      g_stats.launches[p.kind] += 1;
+    }
+    g_stats.pool.push_back(p.start);
+    g_stats.pool.push_back(p.stop);
+  }
+  g_stats.pending.clear();
+}
+
+extern "C" int fla_hip_stats_enable(int on) {
+  g_stats.on = on != 0;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_stats_reset(void) {
+  fla_stats_harvest();
+  for (int i = 0; i < 4; ++i) {
+    g_stats.flops[i] = 0;
+    g_stats.ms[i] = 0;
+    g_stats.launches[i] = 0;
+  }
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_stats_get(int kind, double* flops, double* ms, long* launches) {
+  if (kind < 0 || kind > 3) return FLA_HIP_EARG;
+  fla_stats_harvest();
+  if (flops) *flops = g_stats.flops[kind];
+  if (ms) *ms = g_stats.ms[kind];
+  if (launches) *launches = g_stats.launches[kind];
+  return FLA_HIP_OK;
+}

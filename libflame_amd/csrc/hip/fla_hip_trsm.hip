@@ -1,0 +1,206 @@
+// Triangular solves on the GPU.
+//
+// General dtrsm (all side/uplo/trans/diag combinations of dtrsm_,
+// src/flablas/f2c/dtrsm.c, used by FLA_Trsm_external and the solve paths
+// FLA_Chol_solve / FLA_LU_piv_solve / dpotrs_ / dgetrs_): the diagonal
+// 128x128 blocks of the triangle are inverted in parallel (one workgroup per
+// block), then a block substitution runs as a chain of MFMA updates:
+//   X_k := op(inv(T_kk)) * (B_k - sum_{j<k} op(T)_kj X_j)
+// Each product is an MFMA GEMM whose contracted dimension is one 128-block,
+// so X_k can overwrite B_k in place (every workgroup owns whole rows/columns
+// of the block it writes).
+//
+// LU panel: A12 := inv(tril_unit(L11)) * A12 (FLA_LU_piv_blk_var5.c:88-91,
+// dtrsm_('L','L','N','U')) runs as one lane per column with the 128-entry
+// column in VGPRs and L11 read through uniform loads.
+#include "fla_hip_internal.h"
+
+namespace {
+
+constexpr int TB = 128;       // diagonal block order
+constexpr int TLD = TB + 1;   // LDS leading dimension
+
+// Inverts the nbk x nbk diagonal block k of a triangular matrix (lower or
+// upper storage, unit or non-unit diagonal) into W (ld TB per block, same
+// triangle orientation as A; the other triangle of W is zeroed).
+// The block is loaded as a lower triangle L (transposed for upper storage),
+// inverted in place column by column from the right (dtrti2's order):
+//   L(j,j) := 1/L(j,j);  L(j+1:,j) := -L(j,j) * inv(L)(j+1:,j+1:) * L(j+1:,j)
+__global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* __restrict__ A,
+                                                           long lda, int upper, int unit,
+                                                           double* __restrict__ W) {
+  __shared__ double S[TB * TLD];
+  __shared__ double tmp[TB];
+  const int k = blockIdx.x;
+  const int r0 = k * TB;
+  const int nbk = (n - r0) < TB ? (n - r0) : TB;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < TB * TB; idx += 256) {
+    const int i = idx % TB, j = idx / TB;  // S(i,j) lower
+    double v = 0.0;
+    if (i < nbk && j < nbk && i >= j) {
+      v = upper ? A[(long)(r0 + j) + (long)(r0 + i) * lda] : A[(long)(r0 + i) + (long)(r0 + j) * lda];
+      if (i == j && unit) v = 1.0;
+    }
+    if (i >= nbk && i == j) v = 1.0;  // identity padding
+    S[j * TLD + i] = v;
+  }
+  __syncthreads();
+  for (int j = TB - 1; j >= 0; --j) {
+    const double ajj = 1.0 / S[j * TLD + j];
+    if (t > j && t < TB) tmp[t] = S[j * TLD + t];
+    __syncthreads();
+    if (t > j && t < TB) {
+      // y_t = sum_{c=j+1}^{t} invL(t,c) * L(c,j)
+      double y = 0.0;
+      for (int c = j + 1; c <= t; ++c) y += S[c * TLD + t] * tmp[c];
+      S[j * TLD + t] = -ajj * y;
+    }
+    if (t == 0) S[j * TLD + j] = ajj;
+    __syncthreads();
+  }
+  double* Wk = W + (long)k * TB * TB;
+  for (int idx = t; idx < TB * TB; idx += 256) {
+    const int i = idx % TB, j = idx / TB;  // W(i,j), ld TB
+    double v;
+    if (upper)
+      v = (i <= j) ? S[i * TLD + j] : 0.0;  // inv(U) = inv(L)' with L = U'
+    else
+      v = (i >= j) ? S[j * TLD + i] : 0.0;
+    Wk[(long)i + (long)j * TB] = v;
+  }
+}
+
+// A12 := inv(L11) * A12 with L11 unit lower 128x128 (ld ldl): lane = column.
+__global__ __launch_bounds__(64) void trsm_llnu_kernel(int ncols, const double* __restrict__ L,
+                                                       long ldl, double* __restrict__ B, long ldb,
+                                                       const int* __restrict__ abort_flag) {
+  (void)abort_flag;  // a null pivot does not stop libflame's LU sweep
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= ncols) return;
+  double* col = B + (long)j * ldb;
+  double x[TB];
+#pragma unroll
+  for (int i = 0; i < TB; ++i) x[i] = col[i];
+#pragma unroll
+  for (int c = 0; c < TB; ++c) {
+#pragma unroll
+    for (int i = c + 1; i < TB; ++i) x[i] -= x[c] * L[(long)i + (long)c * ldl];
+  }
+#pragma unroll
+  for (int i = 0; i < TB; ++i) col[i] = x[i];
+}
+
+}  // namespace
+
+int launch_trsm_llnu(hipStream_t s, int ncols, const double* L11, long ldl, double* A12,
+                     long lda, const int* abort_flag) {
+  if (ncols <= 0) return FLA_HIP_OK;
+  hipLaunchKernelGGL(trsm_llnu_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols, L11, ldl,
+                     A12, lda, abort_flag);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+static bool is_trans(int t) { return t == kTrans || t == kConjTrans; }
+
+// Scales B by alpha (m x n) when alpha != 1: one MFMA-free elementwise pass.
+__global__ void dscal_matrix_kernel(int m, int n, double alpha, double* B, long ldb) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)m * n;
+  if (idx >= total) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  double* p = B + (long)i + (long)j * ldb;
+  *p = (alpha == 0.0) ? 0.0 : alpha * (*p);
+}
+
+extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double alpha,
+                             const double* A, int lda, double* B, int ldb) {
+  FLA_TRY(fla_ensure_ready());
+  if ((side != kLeft && side != kRight) || (uplo != kLower && uplo != kUpper)) return FLA_HIP_EARG;
+  if (m < 0 || n < 0) return FLA_HIP_EARG;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  const int na = (side == kLeft) ? m : n;  // order of A
+  if (lda < na || ldb < m) return FLA_HIP_EARG;
+  FlaHipCtx& c = fla_ctx();
+  hipStream_t s = c.s_update;
+  if (alpha != 1.0) {
+    const long total = (long)m * n;
+    hipLaunchKernelGGL(dscal_matrix_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       m, n, alpha, B, (long)ldb);
+    FLA_HIP_CHECK(hipGetLastError());
+    if (alpha == 0.0) return FLA_HIP_OK;
+  }
+  const bool tr = is_trans(trans);
+  const bool upper = (uplo == kUpper);
+  const bool unit = (diag == kUnit);
+  const int nblk = (na + TB - 1) / TB;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace((size_t)nblk * TB * TB * sizeof(double) + 256, &w));
+  double* W = static_cast<double*>(w);
+  hipLaunchKernelGGL(trtri_blocks_kernel, dim3(nblk), dim3(256), 0, s, na, A, (long)lda,
+                     upper ? 1 : 0, unit ? 1 : 0, W);
+  FLA_HIP_CHECK(hipGetLastError());
+  // op(A) is lower when (lower, N) or (upper, T): forward substitution.
+  const bool op_lower = (!upper && !tr) || (upper && tr);
+  auto blk = [&](int idx) { return idx * TB; };
+  if (side == kLeft) {
+    // op(A) X = B.  Forward when op(A) is lower.
+    for (int q = 0; q < nblk; ++q) {
+      const int kb = op_lower ? q : (nblk - 1 - q);
+      const int r0 = blk(kb);
+      const int nbk = (na - r0) < TB ? (na - r0) : TB;
+      // X_k := op(inv(T_kk)) * B_k  (in place, contracted dim = nbk)
+      const double* Wk = W + (long)kb * TB * TB;
+      FLA_TRY(launch_dgemm_mfma(s, tr, false, 0, nbk, n, nbk, 1.0, Wk, TB, B + r0, ldb, 0.0,
+                                B + r0, ldb, nullptr));
+      // B_rest -= op(A)[rest, k] * X_k
+      if (op_lower) {
+        const int rr = r0 + nbk, mr = na - rr;
+        if (mr > 0) {
+          // op(A)(i, c) for i in rest, c in block: N -> A[i + c*lda]; T -> A[c + i*lda]
+          const double* Ap = tr ? A + (long)r0 + (long)rr * lda : A + (long)rr + (long)r0 * lda;
+          FLA_TRY(launch_dgemm_mfma(s, tr, false, 0, mr, n, nbk, -1.0, Ap, lda, B + r0, ldb, 1.0,
+                                    B + rr, ldb, nullptr));
+        }
+      } else {
+        const int mr = r0;  // rows above
+        if (mr > 0) {
+          const double* Ap = tr ? A + (long)r0 : A + (long)r0 * lda;
+          FLA_TRY(launch_dgemm_mfma(s, tr, false, 0, mr, n, nbk, -1.0, Ap, lda, B + r0, ldb, 1.0,
+                                    B, ldb, nullptr));
+        }
+      }
+    }
+  } else {
+    // X op(A) = B.  Forward over columns when op(A) is upper.
+    const bool fwd = !op_lower;
+    for (int q = 0; q < nblk; ++q) {
+      const int kb = fwd ? q : (nblk - 1 - q);
+      const int c0 = blk(kb);
+      const int nbk = (na - c0) < TB ? (na - c0) : TB;
+      const double* Wk = W + (long)kb * TB * TB;
+      double* Bk = B + (long)c0 * ldb;
+      // X_k := B_k * op(inv(T_kk))   (in place, contracted dim = nbk)
+      FLA_TRY(launch_dgemm_mfma(s, false, tr, 0, m, nbk, nbk, 1.0, Bk, ldb, Wk, TB, 0.0, Bk, ldb,
+                                nullptr));
+      if (fwd) {
+        const int cc = c0 + nbk, nr = na - cc;
+        if (nr > 0) {
+          // B[:, rest] -= X_k * op(A)[k, rest];  op(A)(c, j): N -> A[c + j*lda], T -> A[j + c*lda]
+          const double* Ap = tr ? A + (long)cc + (long)c0 * lda : A + (long)c0 + (long)cc * lda;
+          FLA_TRY(launch_dgemm_mfma(s, false, tr, 0, m, nr, nbk, -1.0, Bk, ldb, Ap, lda, 1.0,
+                                    B + (long)cc * ldb, ldb, nullptr));
+        }
+      } else {
+        const int nr = c0;
+        if (nr > 0) {
+          const double* Ap = tr ? A + (long)c0 * lda : A + (long)c0;
+          FLA_TRY(launch_dgemm_mfma(s, false, tr, 0, m, nr, nbk, -1.0, Bk, ldb, Ap, lda, 1.0, B,
+                                    ldb, nullptr));
+        }
+      }
+    }
+  }
+  return FLA_HIP_OK;
+}

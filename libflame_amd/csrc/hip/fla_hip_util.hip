@@ -1,0 +1,275 @@
+// Device-side utilities: counter-based input generators (the same integer
+// hash as the CPU generators, so uniform inputs are bit-identical on host and
+// device), Gram-matrix construction, FP64 residual products, pivot shifts and
+// the general row-interchange kernel behind FLA_Apply_pivots / dlaswp_.
+#include "fla_hip_internal.h"
+
+namespace {
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ unsigned long long rand_u64(unsigned long long seed,
+                                                       unsigned long long idx) {
+  return mix64(mix64(seed + 0x9E3779B97F4A7C15ULL) ^
+               (idx * 0xD1B54A32D192ED03ULL + 0x632BE59BD9B4E019ULL));
+}
+
+__global__ void gen_kernel(unsigned long long seed, int m, int n, double* A, long lda,
+                           int normal) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)m * n;
+  if (idx >= total) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  const unsigned long long e = (unsigned long long)j * (unsigned long long)m + (unsigned long long)i;
+  double v;
+  if (!normal) {
+    v = (double)(rand_u64(seed, e) >> 11) * 0x1.0p-52 - 1.0;
+  } else {
+    const double u1 = ((double)(rand_u64(seed, 2 * e) >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = (double)(rand_u64(seed, 2 * e + 1) >> 11) * 0x1.0p-53;
+    v = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  A[(long)i + (long)j * lda] = v;
+}
+
+__global__ void add_diag_kernel(int n, double shift, double* A, long lda) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[(long)i + (long)i * lda] += shift;
+}
+
+__global__ void scale_rows_kernel(int m, int n, double* A, long lda) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)m * n;
+  if (idx >= total) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  const double s = pow(10.0, -(double)i / (double)(m > n ? m : n));
+  A[(long)i + (long)j * lda] *= s;
+}
+
+__global__ void shift_pivots_kernel(int to_lapack, int m, int* p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) p[i] += to_lapack ? (i + 1) : -(i + 1);
+}
+
+// Row order produced by the interchanges i <-> i + p[i*incp] for i in
+// [k1, k2] (ascending for incp > 0, descending for incp < 0): perm[q] is the
+// source row of final row q.  The interchanges form a chain, so one lane walks
+// them in order; the identity part is filled by the whole grid first.
+__global__ void perm_init_kernel(int m, int* perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) perm[i] = i;
+}
+
+__global__ void perm_walk_kernel(int rows, int k1, int k2, const int* __restrict__ p, int incp,
+                                 int* perm, int* bad) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int step = incp > 0 ? 1 : -1;
+  const int pstride = incp > 0 ? incp : -incp;
+  const int ib = incp > 0 ? k1 : k2;
+  const int ie = incp > 0 ? k2 + 1 : k1 - 1;
+  for (int i = ib; i != ie; i += step) {
+    if (i < 0 || i >= rows) {
+      *bad = 1;
+      return;
+    }
+    const int t = i + p[(long)i * pstride];
+    if (t < 0 || t >= rows) {  // pivot outside the matrix: leave the rest alone
+      *bad = 1;
+      return;
+    }
+    const int a = perm[i];
+    perm[i] = perm[t];
+    perm[t] = a;
+  }
+}
+
+// new(q, j) := old(perm[q], j) for rows q in [0, rows), through a copy.
+__global__ void gather_rows_kernel(int rows, int ncols, const int* __restrict__ perm,
+                                   const double* __restrict__ src, long lds,
+                                   double* __restrict__ dst, long ldd) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)rows * ncols;
+  if (idx >= total) return;
+  const int q = (int)(idx % rows), j = (int)(idx / rows);
+  dst[(long)q + (long)j * ldd] = src[(long)perm[q] + (long)j * lds];
+}
+
+// ---- FP64 reference products for residual checks -----------------------
+// op(M)(i, j) for kind 0 (full), 1 (symmetric from the uplo triangle),
+// 2 (triangular uplo, unit or not).
+struct MatSpec {
+  int kind, lower, trans, unit;
+  const double* M;
+  long ld;
+  __device__ __forceinline__ double get(int i, int j) const {
+    if (trans && kind != 1) {
+      const int t = i;
+      i = j;
+      j = t;
+    }
+    if (kind == 0) return M[(long)i + (long)j * ld];
+    if (kind == 1) {
+      const bool in = lower ? (i >= j) : (i <= j);
+      return in ? M[(long)i + (long)j * ld] : M[(long)j + (long)i * ld];
+    }
+    const bool in = lower ? (i >= j) : (i <= j);
+    if (!in) return 0.0;
+    if (i == j && unit) return 1.0;
+    return M[(long)i + (long)j * ld];
+  }
+};
+
+constexpr int CT = 64;  // tile order of the check product
+
+// Y(i0:i0+64, :) = beta*Y + op(M)(i0:, :) * X, X with nrhs <= 8 columns.
+__global__ __launch_bounds__(256) void matmul_check_kernel(MatSpec S, int n, int nrhs,
+                                                           const double* __restrict__ X, long ldx,
+                                                           double beta, double* __restrict__ Y,
+                                                           long ldy) {
+  __shared__ double T[CT][CT + 1];
+  __shared__ double Xs[CT][8];
+  const int i0 = blockIdx.x * CT;
+  const int t = threadIdx.x;
+  const int ri = t & (CT - 1);  // row within tile
+  const int rg = t >> 6;        // 0..3: rhs group (2 rhs each)
+  double acc[2] = {0.0, 0.0};
+  for (int j0 = 0; j0 < n; j0 += CT) {
+    // Load the tile op(M)(i0.., j0..) choosing a coalesced orientation.
+    for (int e = t; e < CT * CT; e += 256) {
+      const int a = e & (CT - 1), b = e >> 6;
+      // Try "i fastest" (a = row) when the source is M[i + j*ld], else "j fastest".
+      int i = i0 + a, j = j0 + b;
+      double v = 0.0;
+      if (i < n && j < n) v = S.get(i, j);
+      T[a][b] = v;
+    }
+    for (int e = t; e < CT * 8; e += 256) {
+      const int a = e & (CT - 1), r = e >> 6;
+      Xs[a][r] = (j0 + a < n && r < nrhs) ? X[(long)(j0 + a) + (long)r * ldx] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int b = 0; b < CT; ++b) {
+      const double mv = T[ri][b];
+      acc[0] += mv * Xs[b][2 * rg];
+      acc[1] += mv * Xs[b][2 * rg + 1];
+    }
+    __syncthreads();
+  }
+  const int i = i0 + ri;
+  if (i < n) {
+    for (int q = 0; q < 2; ++q) {
+      const int r = 2 * rg + q;
+      if (r < nrhs) {
+        double* y = Y + (long)i + (long)r * ldy;
+        *y = (beta == 0.0) ? acc[q] : acc[q] + beta * (*y);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int fla_hip_gen_uniform(unsigned long long seed, int m, int n, double* A, int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (m <= 0 || n <= 0) return FLA_HIP_OK;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(gen_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     fla_ctx().s_update, seed, m, n, A, (long)lda, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_gen_normal(unsigned long long seed, int m, int n, double* A, int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (m <= 0 || n <= 0) return FLA_HIP_OK;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(gen_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     fla_ctx().s_update, seed, m, n, A, (long)lda, 1);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_gram_lower(int n, int k, const double* B, int ldb, double scale,
+                                  double shift, double* A, int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (n <= 0) return FLA_HIP_OK;
+  hipStream_t s = fla_ctx().s_update;
+  FLA_TRY(launch_dgemm_mfma(s, false, true, 1, n, n, k, scale, B, ldb, B, ldb, 0.0, A, lda,
+                            nullptr));
+  hipLaunchKernelGGL(add_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, shift, A,
+                     (long)lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_scale_rows_logspace(int m, int n, double* A, int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (m <= 0 || n <= 0) return FLA_HIP_OK;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     fla_ctx().s_update, m, n, A, (long)lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dmatmul_check(int kind, int uplo, int trans, int diag, int n, int nrhs,
+                                     const double* M, int ldm, const double* X, int ldx,
+                                     double beta, double* Y, int ldy) {
+  FLA_TRY(fla_ensure_ready());
+  if (n <= 0 || nrhs <= 0) return FLA_HIP_OK;
+  if (nrhs > 8 || kind < 0 || kind > 2) return FLA_HIP_EARG;
+  MatSpec S;
+  S.kind = kind;
+  S.lower = (uplo != kUpper);
+  S.trans = (trans == kTrans || trans == kConjTrans);
+  S.unit = (diag == kUnit);
+  S.M = M;
+  S.ld = ldm;
+  hipLaunchKernelGGL(matmul_check_kernel, dim3((n + CT - 1) / CT), dim3(256), 0,
+                     fla_ctx().s_update, S, n, nrhs, X, (long)ldx, beta, Y, (long)ldy);
+  FLA_HIP_CHECK(hipGetLastError());
+  return fla_hip_synchronize();
+}
+
+extern "C" int fla_hip_shift_pivots(int ptype, int m, int* p) {
+  FLA_TRY(fla_ensure_ready());
+  if (m <= 0) return FLA_HIP_OK;
+  if (ptype != kNativePivots && ptype != kLapackPivots) return FLA_HIP_EARG;
+  hipLaunchKernelGGL(shift_pivots_kernel, dim3((m + 255) / 256), dim3(256), 0, fla_ctx().s_update,
+                     ptype == kLapackPivots ? 1 : 0, m, p);
+  FLA_HIP_CHECK(hipGetLastError());
+  return fla_hip_synchronize();
+}
+
+extern "C" int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, const int* p,
+                              int incp) {
+  FLA_TRY(fla_ensure_ready());
+  if (ncols <= 0 || k2 < k1 || incp == 0) return FLA_HIP_OK;
+  hipStream_t s = fla_ctx().s_update;
+  // The interchanges touch rows up to max(i + p[i]); the permutation is built
+  // over rows [0, lda) (every row index must lie inside the leading dimension).
+  const int rows = lda;
+  void* w = nullptr;
+  const size_t perm_bytes = ((size_t)rows * sizeof(int) + 255) & ~(size_t)255;
+  FLA_TRY(fla_workspace(perm_bytes + (size_t)rows * ncols * sizeof(double), &w));
+  int* perm = static_cast<int*>(w);
+  double* tmp = reinterpret_cast<double*>(static_cast<char*>(w) + perm_bytes);
+  hipLaunchKernelGGL(perm_init_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, rows, perm);
+  int* bad = fla_ctx().dscratch + 8;
+  FLA_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  hipLaunchKernelGGL(perm_walk_kernel, dim3(1), dim3(64), 0, s, rows, k1, k2, p, incp, perm, bad);
+  // tmp := A (rows x ncols), then A(q, :) := tmp(perm[q], :)
+  FLA_HIP_CHECK(hipMemcpy2DAsync(tmp, (size_t)rows * sizeof(double), A, (size_t)lda * sizeof(double),
+                                 (size_t)rows * sizeof(double), (size_t)ncols,
+                                 hipMemcpyDeviceToDevice, s));
+  const long total = (long)rows * ncols;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     rows, ncols, perm, tmp, (long)rows, A, (long)lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}

@@ -1,0 +1,218 @@
+/*
+ * lapack2flame.c -- LAPACK-ABI entry points over libflame_amd.
+ *
+ *   dpotrf_ / dpotf2_  src/map/lapack2flamec/FLA_potrf.c:31-105
+ *                      (LAPACK_potrf_body :38-60, info = idx + 1 :57-58)
+ *   dgetrf_ / dgetf2_  src/map/lapack2flamec/FLA_getrf.c:32-117
+ *                      (LAPACK_getrf_body :40-67: p zeroed, FLA_LU_piv,
+ *                       FLA_Shift_pivots_to(FLA_LAPACK_PIVOTS))
+ *   dpotrs_            src/map/common/lapacksrc/f2c/dpotrs.c:102-194
+ *   dgetrs_            src/map/common/lapacksrc/f2c/dgetrs.c:115-210
+ *
+ * The buffers are host memory (the LAPACK ABI); FLA_Chol / FLA_LU_piv stage
+ * them through device memory.  Argument errors follow the netlib checks
+ * (dpotrf_check / dgetrf_check in src/map/common/lapackcheck): info = -i for
+ * the i-th argument, reported through an xerbla-style message.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "FLAME.h"
+#include "FLA_hip.h"
+#include "fla_internal.h"
+
+static int up(char c) { return (c >= 'a' && c <= 'z') ? c - 32 : c; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static int imin(int a, int b) { return a < b ? a : b; }
+
+static void xerbla(const char* name, int info) {
+  fprintf(stderr, " ** On entry to %s parameter number %d had an illegal value\n", name, info);
+}
+
+static int potrf_body(const char* name, char* uplo, int* n, double* a, int* lda, int* info) {
+  FLA_Obj A;
+  FLA_Error e, init_result;
+  int u = up(*uplo);
+  *info = 0;
+  if (u != 'U' && u != 'L')
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*lda < imax(1, *n))
+    *info = -4;
+  if (*info != 0) {
+    xerbla(name, -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  FLA_Init_safe(&init_result);
+  FLA_Obj_create_without_buffer(FLA_DOUBLE, (dim_t)*n, (dim_t)*n, &A);
+  FLA_Obj_attach_buffer(a, 1, (dim_t)*lda, &A);
+  e = FLA_Chol(u == 'L' ? FLA_LOWER_TRIANGULAR : FLA_UPPER_TRIANGULAR, A);
+  FLA_Obj_free_without_buffer(&A);
+  FLA_Finalize_safe(init_result);
+  *info = (e != FLA_SUCCESS) ? e + 1 : 0;
+  return 0;
+}
+
+int dpotrf_(char* uplo, int* n, double* buff_A, int* ldim_A, int* info) {
+  return potrf_body("DPOTRF", uplo, n, buff_A, ldim_A, info);
+}
+
+int dpotf2_(char* uplo, int* n, double* buff_A, int* ldim_A, int* info) {
+  return potrf_body("DPOTF2", uplo, n, buff_A, ldim_A, info);
+}
+
+static int getrf_body(const char* name, int* m, int* n, double* a, int* lda, int* ipiv,
+                      int* info) {
+  FLA_Obj A, p;
+  FLA_Error e, init_result;
+  int mn;
+  *info = 0;
+  if (*m < 0)
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*lda < imax(1, *m))
+    *info = -4;
+  if (*info != 0) {
+    xerbla(name, -*info);
+    return 0;
+  }
+  if (*m == 0 || *n == 0) return 0;
+  mn = imin(*m, *n);
+  FLA_Init_safe(&init_result);
+  FLA_Obj_create_without_buffer(FLA_DOUBLE, (dim_t)*m, (dim_t)*n, &A);
+  FLA_Obj_attach_buffer(a, 1, (dim_t)*lda, &A);
+  FLA_Obj_create_without_buffer(FLA_INT, (dim_t)mn, 1, &p);
+  FLA_Obj_attach_buffer(ipiv, 1, (dim_t)mn, &p);
+  FLA_Set(FLA_ZERO, p);
+  e = FLA_LU_piv(A, p);
+  FLA_Shift_pivots_to(FLA_LAPACK_PIVOTS, p);
+  FLA_Obj_free_without_buffer(&A);
+  FLA_Obj_free_without_buffer(&p);
+  FLA_Finalize_safe(init_result);
+  *info = (e != FLA_SUCCESS) ? e + 1 : 0;
+  return 0;
+}
+
+int dgetrf_(int* m, int* n, double* buff_A, int* ldim_A, int* buff_p, int* info) {
+  return getrf_body("DGETRF", m, n, buff_A, ldim_A, buff_p, info);
+}
+
+int dgetf2_(int* m, int* n, double* buff_A, int* ldim_A, int* buff_p, int* info) {
+  return getrf_body("DGETF2", m, n, buff_A, ldim_A, buff_p, info);
+}
+
+/* Stages A (n x n) and B (n x nrhs) on the device, runs the two (dpotrs) or
+   pivot + two (dgetrs) triangular solves, and copies B back. */
+int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb,
+            int* info) {
+  int u = up(*uplo), st;
+  double *dA = NULL, *dB = NULL;
+  *info = 0;
+  if (u != 'U' && u != 'L')
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*nrhs < 0)
+    *info = -3;
+  else if (*lda < imax(1, *n))
+    *info = -5;
+  else if (*ldb < imax(1, *n))
+    *info = -7;
+  if (*info != 0) {
+    xerbla("DPOTRS", -*info);
+    return 0;
+  }
+  if (*n == 0 || *nrhs == 0) return 0;
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)*n * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dB, sizeof(double) * (size_t)*n * (size_t)*nrhs);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *n, a, *lda, dA, *n);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *nrhs, b, *ldb, dB, *n);
+  if (st == FLA_HIP_OK) {
+    if (u == 'U') {
+      st = fla_hip_dtrsm(FLA_LEFT, FLA_UPPER_TRIANGULAR, FLA_TRANSPOSE, FLA_NONUNIT_DIAG, *n,
+                         *nrhs, 1.0, dA, *n, dB, *n);
+      if (st == FLA_HIP_OK)
+        st = fla_hip_dtrsm(FLA_LEFT, FLA_UPPER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_NONUNIT_DIAG,
+                           *n, *nrhs, 1.0, dA, *n, dB, *n);
+    } else {
+      st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_NONUNIT_DIAG, *n,
+                         *nrhs, 1.0, dA, *n, dB, *n);
+      if (st == FLA_HIP_OK)
+        st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_TRANSPOSE, FLA_NONUNIT_DIAG, *n,
+                           *nrhs, 1.0, dA, *n, dB, *n);
+    }
+  }
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(*n, *nrhs, dB, *n, b, *ldb);
+  fla_hip_free(dA);
+  fla_hip_free(dB);
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: dpotrs_ failed: %s\n", fla_hip_error_string(st));
+    *info = -100;
+  }
+  return 0;
+}
+
+int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b,
+            int* ldb, int* info) {
+  int t = up(*trans), st, i;
+  double *dA = NULL, *dB = NULL;
+  int* dp = NULL;
+  int* fp = NULL;
+  *info = 0;
+  if (t != 'N' && t != 'T' && t != 'C')
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*nrhs < 0)
+    *info = -3;
+  else if (*lda < imax(1, *n))
+    *info = -5;
+  else if (*ldb < imax(1, *n))
+    *info = -8;
+  if (*info != 0) {
+    xerbla("DGETRS", -*info);
+    return 0;
+  }
+  if (*n == 0 || *nrhs == 0) return 0;
+  /* LAPACK (1-based absolute) pivots -> FLAME-native (0-based relative). */
+  fp = (int*)malloc(sizeof(int) * (size_t)*n);
+  if (!fp) return 0;
+  for (i = 0; i < *n; ++i) fp[i] = ipiv[i] - (i + 1);
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)*n * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dB, sizeof(double) * (size_t)*n * (size_t)*nrhs);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dp, sizeof(int) * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *n, a, *lda, dA, *n);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *nrhs, b, *ldb, dB, *n);
+  if (st == FLA_HIP_OK) st = fla_hip_memcpy_h2d(dp, fp, sizeof(int) * (size_t)*n);
+  if (st == FLA_HIP_OK) {
+    if (t == 'N') {
+      st = fla_hip_dlaswp(*nrhs, dB, *n, 0, *n - 1, dp, 1);
+      if (st == FLA_HIP_OK)
+        st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_UNIT_DIAG, *n,
+                           *nrhs, 1.0, dA, *n, dB, *n);
+      if (st == FLA_HIP_OK)
+        st = fla_hip_dtrsm(FLA_LEFT, FLA_UPPER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_NONUNIT_DIAG,
+                           *n, *nrhs, 1.0, dA, *n, dB, *n);
+    } else {
+      st = fla_hip_dtrsm(FLA_LEFT, FLA_UPPER_TRIANGULAR, FLA_TRANSPOSE, FLA_NONUNIT_DIAG, *n,
+                         *nrhs, 1.0, dA, *n, dB, *n);
+      if (st == FLA_HIP_OK)
+        st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_TRANSPOSE, FLA_UNIT_DIAG, *n,
+                           *nrhs, 1.0, dA, *n, dB, *n);
+      if (st == FLA_HIP_OK) st = fla_hip_dlaswp(*nrhs, dB, *n, 0, *n - 1, dp, -1);
+    }
+  }
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(*n, *nrhs, dB, *n, b, *ldb);
+  fla_hip_free(dA);
+  fla_hip_free(dB);
+  fla_hip_free(dp);
+  free(fp);
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: dgetrs_ failed: %s\n", fla_hip_error_string(st));
+    *info = -100;
+  }
+  return 0;
+}

@@ -1,0 +1,106 @@
+"""CPU-side checks of the drop-in boundary: the C-ABI library loads and
+exports every function that include/*.h declares, the constants keep the
+reference's numeric values, and parameter errors abort as FLA_Check_error_code
+does (no GPU call is made in this file)."""
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "libflame_amd", "libflame_amd.so")
+
+
+def declared_functions():
+    names = set()
+    for h in ("FLAME.h", "FLA_hip.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "while", "for", "return", "sizeof"):
+                names.add(name)
+    return names
+
+
+def test_library_built():
+    assert os.path.exists(LIB), "run __graft_entry__.build() first"
+
+
+def test_exports_every_declared_function():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = sorted(n for n in declared_functions() if n not in exported)
+    assert not missing, f"declared but not exported: {missing}"
+    # the FLA_Obj constants are data symbols
+    for sym in ("FLA_ONE", "FLA_ZERO", "FLA_MINUS_ONE"):
+        assert sym in exported
+
+
+def test_constants_match_reference_values():
+    text = open(os.path.join(ROOT, "include", "FLAME.h")).read()
+    want = {"FLA_SUCCESS": "(-1)", "FLA_FAILURE": "(-2)", "FLA_LOWER_TRIANGULAR": "300",
+            "FLA_UPPER_TRIANGULAR": "301", "FLA_NATIVE_PIVOTS": "700", "FLA_LAPACK_PIVOTS": "701",
+            "FLA_DOUBLE": "101", "FLA_INT": "104", "FLA_LEFT": "210", "FLA_NO_TRANSPOSE": "400",
+            "FLA_UNIT_DIAG": "500", "FLA_INVALID_UPLO": "(-11)",
+            "FLA_OBJECT_NOT_SQUARE": "(-24)"}
+    for k, v in want.items():
+        assert re.search(rf"#define {k} {re.escape(v)}(?![\w(])", text), k
+
+
+def test_python_binding_loads_without_gpu():
+    import libflame_amd as fla
+    L = fla.lib()
+    L.FLA_Init()
+    assert L.FLA_Initialized() == 1
+    L.FLA_Finalize()
+
+
+_ABORT_SNIPPET = r"""
+import ctypes as C, sys
+sys.path.insert(0, {root!r})
+import numpy as np
+import libflame_amd as fla
+L = fla.lib()
+L.FLA_Init()
+o = fla.FLA_Obj()
+L.FLA_Obj_create_without_buffer(fla.FLA_DOUBLE, 4, 3, C.byref(o))
+buf = np.zeros((4, 3), order="F")
+o.base.contents.buffer = buf.ctypes.data
+o.base.contents.rs, o.base.contents.cs = 1, 4
+{call}
+print("returned")
+"""
+
+
+@pytest.mark.parametrize("call", ["L.FLA_Chol(fla.FLA_LOWER_TRIANGULAR, o)",  # not square
+                                  "L.FLA_Chol(999, o)"])                     # bad uplo
+def test_parameter_errors_abort(call):
+    """FLA_Chol_check fails -> message + abort() (SIGABRT), before any GPU use."""
+    code = _ABORT_SNIPPET.format(root=ROOT, call=call)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == -6, (r.returncode, r.stdout, r.stderr)
+    assert "libflame:" in r.stderr
+    assert "returned" not in r.stdout
+
+
+def test_error_checking_disabled_skips_abort():
+    """With FLA_NO_ERROR_CHECKING the check is skipped (FLA_Chol.c:20-22); a
+    zero-size object then returns FLA_SUCCESS without touching the GPU."""
+    code = r"""
+import ctypes as C, sys
+sys.path.insert(0, %r)
+import libflame_amd as fla
+L = fla.lib()
+L.FLA_Init()
+L.FLA_Check_error_level_set(fla.FLA_NO_ERROR_CHECKING)
+o = fla.FLA_Obj()
+L.FLA_Obj_create_without_buffer(fla.FLA_DOUBLE, 0, 0, C.byref(o))
+print(L.FLA_Chol(999, o))
+""" % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "-1"
