@@ -134,9 +134,9 @@ __global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
   double ra[8], rb[8];
   if (nk > 0) {
     load_chunk<TA>(A, lda, m, k, i0, 0, ra, tid);
-    load_chunk<TB>(B, ldb, n, k, j0, 0, rb, tid);
+    load_chunk<!TB>(B, ldb, n, k, j0, 0, rb, tid);
     store_chunk<TA>(As[0], ra, tid);
-    store_chunk<TB>(Bs[0], rb, tid);
+    store_chunk<!TB>(Bs[0], rb, tid);
   }
   __syncthreads();
 
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
     const bool more = kt + 1 < nk;
     if (more) {
       load_chunk<TA>(A, lda, m, k, i0, (kt + 1) * BK, ra, tid);
-      load_chunk<TB>(B, ldb, n, k, j0, (kt + 1) * BK, rb, tid);
+      load_chunk<!TB>(B, ldb, n, k, j0, (kt + 1) * BK, rb, tid);
     }
     const double* as = As[cur];
     const double* bs = Bs[cur];
@@ -166,12 +166,14 @@ __global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
     }
     if (more) {
       store_chunk<TA>(As[cur ^ 1], ra, tid);
-      store_chunk<TB>(Bs[cur ^ 1], rb, tid);
+      store_chunk<!TB>(Bs[cur ^ 1], rb, tid);
     }
     __syncthreads();
   }
 
-  // Epilogue: lane holds C(i0+wr*64+a*16+li, j0+wc*64+b*16+4*lk+r) in acc[a][b][r].
+  // Epilogue.  v_mfma_f64_16x16x4f64 leaves D(x, y) in lane 16*(x%4) + y,
+  // register x/4 (measured on gfx950); with x = column, y = row here, lane
+  // holds C(i0+wr*64+a*16+li, j0+wc*64+b*16+lk+4*r) in acc[a][b][r].
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int i = i0 + wr * 64 + a * 16 + li;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
     for (int b = 0; b < 4; ++b) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = j0 + wc * 64 + b * 16 + 4 * lk + r;
+        const int j = j0 + wc * 64 + b * 16 + lk + 4 * r;
         if (j >= n) continue;
         if (mode == 1 && i < j) continue;
         double* c = C + (long)i + (long)j * ldc;

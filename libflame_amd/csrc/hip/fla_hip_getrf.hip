@@ -549,12 +549,7 @@ static int lu_update_cols(hipStream_t s, int m, double* A, long lda, int k, int 
   FLA_TRY(lu_gather(s, nc, Ak + (long)c0 * lda, lda, moved));
   double* A11 = A + (long)k + (long)k * lda;
   double* A12 = A + (long)k + (long)c0 * lda;
-  if (kb == PW) {
-    FLA_TRY(launch_trsm_llnu(s, nc, A11, lda, A12, lda, nullptr));
-  } else {
-    FLA_TRY(fla_hip_dtrsm(kLeft, kLower, kNoTrans, kUnit, kb, nc, 1.0, A11, (int)lda, A12,
-                          (int)lda));
-  }
+  FLA_TRY(launch_trsm_llnu(s, nc, kb, A11, lda, A12, lda));
   const int mr = m - k - kb;
   if (mr > 0) {
     void* tok = fla_stats_begin(s);

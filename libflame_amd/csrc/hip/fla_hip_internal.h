@@ -86,8 +86,8 @@ int launch_potrf_diag(hipStream_t s, int nb, double* A, long lda, int col_offset
 // A21 := A21 * inv(L11)' for an m x nb strip, L11 lower non-unit (nb <= 128).
 int launch_trsm_rlt(hipStream_t s, int m, int nb, const double* L11, long ldl, double* A21,
                     long lda, const int* abort_flag);
-// A12 := inv(L11) * A12, L11 unit lower 128 x 128: one lane per column.
-int launch_trsm_llnu(hipStream_t s, int ncols, const double* L11, long ldl, double* A12,
-                     long lda, const int* abort_flag);
+// A12 := inv(L11) * A12, L11 unit lower nb x nb (nb <= 128): one lane per column.
+int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long ldl, double* A12,
+                     long lda);
 // In-place exchange of the strictly lower and strictly upper triangles.
 int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

@@ -71,33 +71,44 @@ __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* 
   }
 }
 
-// A12 := inv(L11) * A12 with L11 unit lower 128x128 (ld ldl): lane = column.
-__global__ __launch_bounds__(64) void trsm_llnu_kernel(int ncols, const double* __restrict__ L,
-                                                       long ldl, double* __restrict__ B, long ldb,
-                                                       const int* __restrict__ abort_flag) {
-  (void)abort_flag;  // a null pivot does not stop libflame's LU sweep
+// A12 := inv(L11) * A12 with L11 unit lower nb x nb (nb <= 128, ld ldl):
+// lane = column, forward substitution in dtrsm_('L','L','N','U') order.
+// FULL (nb == 128) drops the uniform bounds tests of the partial last panel.
+template <bool FULL>
+__global__ __launch_bounds__(64) void trsm_llnu_kernel(int ncols, int nb,
+                                                       const double* __restrict__ L, long ldl,
+                                                       double* __restrict__ B, long ldb) {
   const int j = blockIdx.x * 64 + threadIdx.x;
   if (j >= ncols) return;
   double* col = B + (long)j * ldb;
   double x[TB];
 #pragma unroll
-  for (int i = 0; i < TB; ++i) x[i] = col[i];
+  for (int i = 0; i < TB; ++i) x[i] = (FULL || i < nb) ? col[i] : 0.0;
 #pragma unroll
   for (int c = 0; c < TB; ++c) {
+    if (FULL || c < nb) {
 #pragma unroll
-    for (int i = c + 1; i < TB; ++i) x[i] -= x[c] * L[(long)i + (long)c * ldl];
+      for (int i = c + 1; i < TB; ++i)
+        if (FULL || i < nb) x[i] -= x[c] * L[(long)i + (long)c * ldl];
+    }
   }
 #pragma unroll
-  for (int i = 0; i < TB; ++i) col[i] = x[i];
+  for (int i = 0; i < TB; ++i)
+    if (FULL || i < nb) col[i] = x[i];
 }
 
 }  // namespace
 
-int launch_trsm_llnu(hipStream_t s, int ncols, const double* L11, long ldl, double* A12,
-                     long lda, const int* abort_flag) {
-  if (ncols <= 0) return FLA_HIP_OK;
-  hipLaunchKernelGGL(trsm_llnu_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols, L11, ldl,
-                     A12, lda, abort_flag);
+int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long ldl, double* A12,
+                     long lda) {
+  if (ncols <= 0 || nb <= 0) return FLA_HIP_OK;
+  if (nb > TB) return FLA_HIP_EARG;
+  if (nb == TB)
+    hipLaunchKernelGGL(trsm_llnu_kernel<true>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols, nb,
+                       L11, ldl, A12, lda);
+  else
+    hipLaunchKernelGGL(trsm_llnu_kernel<false>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols,
+                       nb, L11, ldl, A12, lda);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
