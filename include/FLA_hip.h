@@ -88,6 +88,8 @@ int fla_hip_shift_pivots(int ptype, int m, int* p);
 /* Outer (trailing-update) and inner (panel) block sizes; 0 keeps current. */
 int fla_hip_set_blocksizes(int chol_nb, int lu_nb);
 int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb);
+/* Tile configuration of the MFMA update kernel (0 = default). */
+int fla_hip_set_gemm_config(int cfg);
 
 /* ---- measurement ---- */
 /* When enabled, the factorization drivers bracket every trailing-update

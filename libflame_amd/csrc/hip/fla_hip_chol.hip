@@ -14,91 +14,148 @@
 
 namespace {
 
-constexpr int PB = 128;         // panel sub-block width (diagonal block order)
-constexpr int PLD = PB + 1;     // LDS leading dimension of the diagonal block
-constexpr int POTRF_THREADS = 1024;
+constexpr int PB = 128;  // panel sub-block width (diagonal block order)
+constexpr int LB = 32;   // leaf width of the panel triangular solve
 
-// Single-workgroup Cholesky of the nb x nb lower block at A (nb <= 128), kept
-// in LDS.  Right-looking, one column per step: the pivot check and sqrt are
-// computed redundantly by every thread (identical values, no extra barrier),
-// the scaled column goes through a separate LDS vector so that the trailing
-// update reads unscaled-free values, and two barriers separate the steps.
-__global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(int nb, double* __restrict__ A,
-                                                                    long lda, int col_offset,
-                                                                    int* __restrict__ info) {
+// Single-workgroup Cholesky of the nb x nb lower block at A (nb <= 128).
+//
+// Every thread owns a 4x4 tile of the lower triangle in registers (528
+// tiles; the padding beyond nb is the identity).  Step k (right-looking, the
+// order of dsyrk/dtrsm-free unblocked Cholesky):
+//   d = sqrt(a_kk)  -- failure when a_kk <= 0 or NaN, as bl1_dsqrte
+//                      (src/base/flamec/include/blis_macro_defs.h:134-136)
+//   l_ik = a_ik * (1/d)  (i > k),  a_ic -= l_ik l_ck  (c > k, i >= c)
+// The diagonal entry of the next step and the scaled column travel through
+// LDS; two barriers per step.  The first failing column is stored in *info
+// (offset by col_offset) and the columns before it are written back.
+__global__ __launch_bounds__(1024) void potrf_tile_kernel(int nb, double* __restrict__ A,
+                                                          long lda, int col_offset,
+                                                          int* __restrict__ info) {
   if (*info >= 0) return;
-  __shared__ double S[PB * PLD];
   __shared__ double lcol[PB];
+  __shared__ double dg[2];
   __shared__ int fail;
   const int t = threadIdx.x;
-  if (t == 0) fail = -1;
-  for (int idx = t; idx < nb * nb; idx += POTRF_THREADS) {
-    const int i = idx % nb, c = idx / nb;
-    S[c * PLD + i] = (i >= c) ? A[(long)i + (long)c * lda] : 0.0;
+  constexpr int NTILE = (PB / 4) * (PB / 4 + 1) / 2;  // 528
+  const bool own = t < NTILE;
+  int I = 0;
+  if (own) {
+    I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > t) --I;
+    while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  }
+  const int J = own ? t - I * (I + 1) / 2 : 0;
+  const int r0 = 4 * I, c0 = 4 * J;
+  double a[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int i = r0 + x, c = c0 + y;
+      double v = 0.0;
+      if (own && i >= c) {
+        if (i < nb && c < nb)
+          v = A[(long)i + (long)c * lda];
+        else if (i == c)
+          v = 1.0;
+      }
+      a[x][y] = v;
+    }
+  if (t == 0) {
+    fail = -1;
+    dg[0] = a[0][0];
   }
   __syncthreads();
-  const int i = t & (PB - 1);
-  const int c0 = t >> 7;  // 0..7
-  int j = 0;
-  for (; j < nb; ++j) {
-    const double d2 = S[j * PLD + j];
-    if (!(d2 > 0.0)) {  // <= 0 or NaN, as bl1_dsqrte
-      if (t == 0) fail = j;
+  int k = 0;
+  for (; k < PB; ++k) {
+    const int par = k & 1;
+    const double d2 = dg[par];
+    if (!(d2 > 0.0)) {
+      if (t == 0) fail = k;
       break;
     }
     const double d = sqrt(d2);
     const double inv = 1.0 / d;
-    if (t < nb && t > j) lcol[t] = S[j * PLD + t] * inv;
+    if (own && c0 <= k && k < c0 + 4) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int i = r0 + x;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          if (c0 + y != k) continue;
+          if (i > k) {
+            a[x][y] *= inv;
+            lcol[i] = a[x][y];
+          } else if (i == k) {
+            a[x][y] = d;
+          }
+        }
+      }
+    }
     __syncthreads();
-    if (t == 0) S[j * PLD + j] = d;
-    if (t < nb && t > j) S[j * PLD + t] = lcol[t];
-    if (i < nb && i > j) {
-      const double li = lcol[i];
-#pragma unroll 4
-      for (int s = 0; s < PB / 8; ++s) {
-        const int c = c0 + 8 * s;
-        if (c > j && c <= i) S[c * PLD + i] -= li * lcol[c];
+    if (own) {
+      const int kn = k + 1;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int i = r0 + x;
+        if (i <= k) continue;
+        const double li = lcol[i];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          const int c = c0 + y;
+          if (c > k && i >= c) {
+            a[x][y] -= li * lcol[c];
+            if (i == kn && c == kn) dg[par ^ 1] = a[x][y];
+          }
+        }
       }
     }
     __syncthreads();
   }
-  __syncthreads();
   const int ncols = (fail >= 0) ? fail : nb;
-  for (int idx = t; idx < nb * nb; idx += POTRF_THREADS) {
-    const int ii = idx % nb, c = idx / nb;
-    if (ii >= c && c < ncols) A[(long)ii + (long)c * lda] = S[c * PLD + ii];
+  if (own) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int i = r0 + x, c = c0 + y;
+        if (i < nb && c < ncols && i >= c) A[(long)i + (long)c * lda] = a[x][y];
+      }
   }
   if (t == 0 && fail >= 0) *info = col_offset + fail;
 }
 
-// A21 := A21 * inv(L11)' for an m x 128 strip: one lane per row, the row held
-// in 128 VGPR pairs, L11 read through uniform (scalar) loads.  Same update
-// order as dtrsm_('R','L','T','N'): x_j *= 1/l_jj, then x_c -= l_cj * x_j.
-__global__ __launch_bounds__(64) void trsm_rlt_kernel(int m, const double* __restrict__ L,
-                                                      long ldl, double* __restrict__ A21,
-                                                      long lda, const int* __restrict__ abort_flag) {
+// Leaf of the panel solve: X := X * inv(L)' for an m x nb strip (nb <= 32),
+// L lower non-unit.  One lane per row with the row in VGPRs, L through
+// uniform loads; the update order of dtrsm_('R','L','T','N')
+// (src/flablas/f2c/dtrsm.c): x_j *= 1/l_jj, then x_c -= l_cj * x_j.
+template <bool FULL>
+__global__ __launch_bounds__(64) void trsm_rlt_leaf_kernel(int m, int nb,
+                                                           const double* __restrict__ L, long ldl,
+                                                           double* __restrict__ X, long ldx,
+                                                           const int* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag >= 0) return;
   const int r = blockIdx.x * 64 + threadIdx.x;
-  const bool active = r < m;
-  double x[PB];
+  if (r >= m) return;
+  double x[LB];
 #pragma unroll
-  for (int c = 0; c < PB; ++c) x[c] = active ? A21[(long)r + (long)c * lda] : 0.0;
+  for (int c = 0; c < LB; ++c) x[c] = (FULL || c < nb) ? X[(long)r + (long)c * ldx] : 0.0;
 #pragma unroll
-  for (int j = 0; j < PB; ++j) {
-    const double inv = 1.0 / L[(long)j + (long)j * ldl];
-    x[j] *= inv;
+  for (int j = 0; j < LB; ++j) {
+    if (FULL || j < nb) {
+      x[j] *= 1.0 / L[(long)j + (long)j * ldl];
 #pragma unroll
-    for (int c = j + 1; c < PB; ++c) x[c] -= L[(long)c + (long)j * ldl] * x[j];
+      for (int c = j + 1; c < LB; ++c)
+        if (FULL || c < nb) x[c] -= L[(long)c + (long)j * ldl] * x[j];
+    }
   }
-  if (active) {
 #pragma unroll
-    for (int c = 0; c < PB; ++c) A21[(long)r + (long)c * lda] = x[c];
-  }
+  for (int c = 0; c < LB; ++c)
+    if (FULL || c < nb) X[(long)r + (long)c * ldx] = x[c];
 }
 
 // Exchanges A(i,j) and A(j,i) for all i > j, 32x32 tile pairs through LDS.
-__global__ __launch_bounds__(256) void transpose_swap_kernel(int n, double* __restrict__ A, long lda,
-                                                             int T) {
+__global__ __launch_bounds__(256) void transpose_swap_kernel(int n, double* __restrict__ A, long lda) {
   __shared__ double P[32][33];
   __shared__ double Q[32][33];
   // Lower tile (I, J), I >= J, enumerated row by row.
@@ -117,7 +174,7 @@ __global__ __launch_bounds__(256) void transpose_swap_kernel(int n, double* __re
   }
   __syncthreads();
   for (int y = ty; y < 32; y += 8) {
-    // new lower (ri+tx, cj+y) := old upper (cj+y, ri+tx) = Q[ri+tx - ri][cj+y - cj]
+    // new lower (ri+tx, cj+y) := old upper (cj+y, ri+tx) = Q[tx][y]
     const int i = ri + tx, j = cj + y;
     if (i < n && j < n && i > j) A[(long)i + (long)j * lda] = Q[tx][y];
     const int i2 = cj + tx, j2 = ri + y;
@@ -132,19 +189,37 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 int launch_potrf_diag(hipStream_t s, int nb, double* A, long lda, int col_offset, int* info) {
   if (nb <= 0) return FLA_HIP_OK;
   if (nb > PB) return FLA_HIP_EARG;
-  hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(POTRF_THREADS), 0, s, nb, A, lda,
-                     col_offset, info);
+  hipLaunchKernelGGL(potrf_tile_kernel, dim3(1), dim3(1024), 0, s, nb, A, lda, col_offset, info);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
 
+// A21 := A21 * inv(L11)' for an m x nb strip, L11 lower non-unit (nb <= 128):
+// blocked substitution -- leaves of 32 columns solved in registers, each
+// followed by an MFMA update of the strip's remaining columns.
 int launch_trsm_rlt(hipStream_t s, int m, int nb, const double* L11, long ldl, double* A21,
                     long lda, const int* abort_flag) {
-  if (m <= 0) return FLA_HIP_OK;
-  if (nb != PB) return FLA_HIP_EARG;
-  hipLaunchKernelGGL(trsm_rlt_kernel, dim3((m + 63) / 64), dim3(64), 0, s, m, L11, ldl, A21,
-                     lda, abort_flag);
-  FLA_HIP_CHECK(hipGetLastError());
+  if (m <= 0 || nb <= 0) return FLA_HIP_OK;
+  if (nb > PB) return FLA_HIP_EARG;
+  for (int b0 = 0; b0 < nb; b0 += LB) {
+    const int w = (nb - b0) < LB ? (nb - b0) : LB;
+    const double* Lbb = L11 + (long)b0 + (long)b0 * ldl;
+    double* Xb = A21 + (long)b0 * lda;
+    if (w == LB)
+      hipLaunchKernelGGL(trsm_rlt_leaf_kernel<true>, dim3((m + 63) / 64), dim3(64), 0, s, m, w,
+                         Lbb, ldl, Xb, lda, abort_flag);
+    else
+      hipLaunchKernelGGL(trsm_rlt_leaf_kernel<false>, dim3((m + 63) / 64), dim3(64), 0, s, m, w,
+                         Lbb, ldl, Xb, lda, abort_flag);
+    FLA_HIP_CHECK(hipGetLastError());
+    const int rest = nb - b0 - w;
+    if (rest > 0) {
+      // X[:, b0+w:] -= X_b * L11[b0+w:, b0:b0+w]'
+      FLA_TRY(launch_dgemm_mfma(s, false, true, 0, m, rest, w, -1.0, Xb, lda,
+                                L11 + (long)(b0 + w) + (long)b0 * ldl, ldl, 1.0,
+                                Xb + (long)w * lda, lda, abort_flag));
+    }
+  }
   return FLA_HIP_OK;
 }
 
@@ -152,14 +227,14 @@ int launch_transpose_swap(hipStream_t s, int n, double* A, long lda) {
   if (n <= 1) return FLA_HIP_OK;
   const int T = (n + 31) / 32;
   const long nblk = (long)T * (T + 1) / 2;
-  hipLaunchKernelGGL(transpose_swap_kernel, dim3((unsigned)nblk), dim3(256), 0, s, n, A, lda, T);
+  hipLaunchKernelGGL(transpose_swap_kernel, dim3((unsigned)nblk), dim3(256), 0, s, n, A, lda);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
 
-// Factors columns [k, k+kb) of the lower matrix (rows k..n-1): sub-blocks of
-// 128 = diagonal factor, strip TRSM of everything below, and the update of the
-// remaining panel columns.  All on stream s.
+// Factors columns [k, k+kb) of the lower matrix (rows k..n-1): per 128-column
+// sub-block, the diagonal factor, the blocked panel solve of everything
+// below it, and the update of the panel's remaining columns.  All on stream s.
 static int chol_panel(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info) {
   for (int kk = k; kk < k + kb; kk += PB) {
     const int ib = (k + kb - kk) < PB ? (k + kb - kk) : PB;

@@ -21,73 +21,49 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int BM = 128;
-constexpr int BN = 128;
-constexpr int BK = 16;
-constexpr int NTHR = 256;
-constexpr int LS = BM + 1;  // LDS row stride (doubles); the pad spreads transposed stores
-
-// Loads the 128 x 16 chunk of op(X) starting at (r0, k0) into registers.
-//   T == false: op(X)(r,l) = X[r + l*ld]   (rows contiguous)
-//   T == true : op(X)(r,l) = X[l + r*ld]   (k contiguous)
-template <bool T>
-__device__ __forceinline__ void load_chunk(const double* X, long ld, int rows,
-                                           int kdim, int r0, int k0, double (&reg)[8], int tid) {
+// Loads the BM x BK chunk of op(X) starting at (r0, k0) into registers
+// (NT threads, PER = BM*BK/NT values each).
+//   T == false: op(X)(r,l) = X[r + l*ld]   (rows contiguous: lanes walk r)
+//   T == true : op(X)(r,l) = X[l + r*ld]   (k contiguous: lanes walk l)
+template <bool T, int BM, int BK, int NT>
+__device__ __forceinline__ void load_chunk(const double* X, long ld, int rows, int kdim, int r0,
+                                           int k0, double (&reg)[BM * BK / NT], int tid) {
+  constexpr int PER = BM * BK / NT;
   const bool full = (r0 + BM <= rows) && (k0 + BK <= kdim);
-  if (!T) {
-    const int r = tid & (BM - 1);
-    const int lb = tid >> 7;  // 0..1
-    const double* p = X + (long)(r0 + r) + (long)(k0 + lb) * ld;
-    if (full) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) reg[q] = p[(long)(2 * q) * ld];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int l = lb + 2 * q;
-        reg[q] = (r0 + r < rows && k0 + l < kdim) ? p[(long)(2 * q) * ld] : 0.0;
-      }
-    }
-  } else {
-    const int l = tid & (BK - 1);
-    const int rb = tid >> 4;  // 0..15
-    const double* p = X + (long)(k0 + l) + (long)(r0 + rb) * ld;
-    if (full) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) reg[q] = p[(long)(16 * q) * ld];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int r = rb + 16 * q;
-        reg[q] = (r0 + r < rows && k0 + l < kdim) ? p[(long)(16 * q) * ld] : 0.0;
-      }
-    }
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + q * NT;
+    const int r = T ? (e / BK) : (e % BM);
+    const int l = T ? (e % BK) : (e / BM);
+    const double* p = T ? X + (long)(k0 + l) + (long)(r0 + r) * ld
+                        : X + (long)(r0 + r) + (long)(k0 + l) * ld;
+    if (full)
+      reg[q] = *p;
+    else
+      reg[q] = (r0 + r < rows && k0 + l < kdim) ? *p : 0.0;
   }
 }
 
-// Stores registers from load_chunk into LDS laid out as S[l][r] (stride LS).
-template <bool T>
-__device__ __forceinline__ void store_chunk(double* S, const double (&reg)[8], int tid) {
-  if (!T) {
-    const int r = tid & (BM - 1);
-    const int lb = tid >> 7;
+// Stores registers from load_chunk into LDS laid out as S[l][r] (stride BM+1).
+template <bool T, int BM, int BK, int NT>
+__device__ __forceinline__ void store_chunk(double* S, const double (&reg)[BM * BK / NT], int tid) {
+  constexpr int PER = BM * BK / NT;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) S[(lb + 2 * q) * LS + r] = reg[q];
-  } else {
-    const int l = tid & (BK - 1);
-    const int rb = tid >> 4;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) S[l * LS + rb + 16 * q] = reg[q];
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + q * NT;
+    const int r = T ? (e / BK) : (e % BM);
+    const int l = T ? (e % BK) : (e / BM);
+    S[l * (BM + 1) + r] = reg[q];
   }
 }
 
 // Maps a linear workgroup id to a C tile.  Workgroups are dealt to the eight
 // XCDs round-robin; the id is first permuted so that each XCD receives a
 // contiguous run of tiles (which share operand panels in its own L2).
-// mode 0: column-major over a Tm x Tn grid; mode 1: lower tiles (ti >= tj) of
-// a Tm x Tn grid, column by column.
-__device__ __forceinline__ void tile_of(int bid, int nblk, int mode, int Tm, int Tn, int& ti,
-                                        int& tj) {
+// mode 0: column-major over a Tm x Tn grid; mode 1: lower tiles (whose rows
+// reach the diagonal) of a Tm x Tn grid, column by column.
+__device__ __forceinline__ void tile_of(int bid, int nblk, int mode, int Tm, int Tn, int rpc,
+                                        int& ti, int& tj) {
   const int x = bid & 7;
   const int q = nblk >> 3, rr = nblk & 7;
   const int t = x * q + (x < rr ? x : rr) + (bid >> 3);
@@ -95,48 +71,58 @@ __device__ __forceinline__ void tile_of(int bid, int nblk, int mode, int Tm, int
     tj = t / Tm;
     ti = t - tj * Tm;
   } else {
-    // offset(c) = c*Tm - c*(c-1)/2 tiles precede column c.
-    const double a = 2.0 * Tm + 1.0;
-    int c = (int)((a - sqrt(a * a - 8.0 * (double)t)) * 0.5);
+    // Column c holds the row tiles [c*rpc, Tm): offset(c) = c*Tm - rpc*c*(c-1)/2.
+    int c = 0;
+    {
+      const double a = 2.0 * Tm + rpc;
+      const double disc = a * a - 8.0 * rpc * (double)t;
+      c = (int)((a - sqrt(disc > 0.0 ? disc : 0.0)) / (2.0 * rpc));
+    }
     if (c < 0) c = 0;
     if (c > Tn - 1) c = Tn - 1;
-    while (c > 0 && (long)c * Tm - (long)c * (c - 1) / 2 > t) --c;
-    while (c + 1 < Tn && (long)(c + 1) * Tm - (long)(c + 1) * c / 2 <= t) ++c;
+    while (c > 0 && (long)c * Tm - (long)rpc * c * (c - 1) / 2 > t) --c;
+    while (c + 1 < Tn && (long)(c + 1) * Tm - (long)rpc * (c + 1) * c / 2 <= t) ++c;
     tj = c;
-    ti = c + (t - (c * Tm - c * (c - 1) / 2));
+    ti = c * rpc + (t - (c * Tm - rpc * c * (c - 1) / 2));
   }
 }
 
-template <bool TA, bool TB>
-__global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
-    int mode, int m, int n, int k, double alpha, const double* A, long lda,
-    const double* B, long ldb, double beta, double* C, long ldc,
-    int Tm, int Tn, const int* __restrict__ abort_flag) {
+// C := alpha*op(A)*op(B) + beta*C on a BM x BN tile per workgroup of
+// WM x WN waves; each wave owns (BM/WM) x (BN/WN) = TM x TN MFMA blocks.
+template <bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void dgemm_mfma_kernel(
+    int mode, int m, int n, int k, double alpha, const double* A, long lda, const double* B,
+    long ldb, double beta, double* C, long ldc, int Tm, int Tn, int rpc,
+    const int* __restrict__ abort_flag) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  constexpr int LA = BM + 1, LB = BN + 1;
+  constexpr int PA = BM * BK / NT, PB = BN * BK / NT;
   if (abort_flag && *abort_flag >= 0) return;
-  __shared__ double As[2][BK * LS];
-  __shared__ double Bs[2][BK * LS];
+  __shared__ double As[2][BK * LA];
+  __shared__ double Bs[2][BK * LB];
 
   int ti, tj;
-  tile_of(blockIdx.x, gridDim.x, mode, Tm, Tn, ti, tj);
+  tile_of(blockIdx.x, gridDim.x, mode, Tm, Tn, rpc, ti, tj);
   const int i0 = ti * BM, j0 = tj * BN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave / WN, wc = wave % WN;
   const int li = lane & 15, lk = lane >> 4;
 
-  v4d acc[4][4];
+  v4d acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < TN; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
 
   const int nk = (k + BK - 1) / BK;
-  double ra[8], rb[8];
+  double ra[PA], rb[PB];
   if (nk > 0) {
-    load_chunk<TA>(A, lda, m, k, i0, 0, ra, tid);
-    load_chunk<!TB>(B, ldb, n, k, j0, 0, rb, tid);
-    store_chunk<TA>(As[0], ra, tid);
-    store_chunk<!TB>(Bs[0], rb, tid);
+    load_chunk<TA, BM, BK, NT>(A, lda, m, k, i0, 0, ra, tid);
+    load_chunk<!TB, BN, BK, NT>(B, ldb, n, k, j0, 0, rb, tid);
+    store_chunk<TA, BM, BK, NT>(As[0], ra, tid);
+    store_chunk<!TB, BN, BK, NT>(Bs[0], rb, tid);
   }
   __syncthreads();
 
@@ -144,45 +130,43 @@ __global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
     if (more) {
-      load_chunk<TA>(A, lda, m, k, i0, (kt + 1) * BK, ra, tid);
-      load_chunk<!TB>(B, ldb, n, k, j0, (kt + 1) * BK, rb, tid);
+      load_chunk<TA, BM, BK, NT>(A, lda, m, k, i0, (kt + 1) * BK, ra, tid);
+      load_chunk<!TB, BN, BK, NT>(B, ldb, n, k, j0, (kt + 1) * BK, rb, tid);
     }
     const double* as = As[cur];
     const double* bs = Bs[cur];
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
-      const int row = (kk * 4 + lk) * LS;
-      double fa[4], fb[4];
+      double fa[TM], fb[TN];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        fa[t] = as[row + wr * 64 + t * 16 + li];
-        fb[t] = bs[row + wc * 64 + t * 16 + li];
-      }
+      for (int t = 0; t < TM; ++t) fa[t] = as[(kk * 4 + lk) * LA + wr * (BM / WM) + t * 16 + li];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int t = 0; t < TN; ++t) fb[t] = bs[(kk * 4 + lk) * LB + wc * (BN / WN) + t * 16 + li];
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[b], fa[a], acc[a][b], 0, 0, 0);
     }
     if (more) {
-      store_chunk<TA>(As[cur ^ 1], ra, tid);
-      store_chunk<!TB>(Bs[cur ^ 1], rb, tid);
+      store_chunk<TA, BM, BK, NT>(As[cur ^ 1], ra, tid);
+      store_chunk<!TB, BN, BK, NT>(Bs[cur ^ 1], rb, tid);
     }
     __syncthreads();
   }
 
   // Epilogue.  v_mfma_f64_16x16x4f64 leaves D(x, y) in lane 16*(x%4) + y,
   // register x/4 (measured on gfx950); with x = column, y = row here, lane
-  // holds C(i0+wr*64+a*16+li, j0+wc*64+b*16+lk+4*r) in acc[a][b][r].
+  // holds C(i0+wr*BM/WM+a*16+li, j0+wc*BN/WN+b*16+lk+4*r) in acc[a][b][r].
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = i0 + wr * 64 + a * 16 + li;
+  for (int a = 0; a < TM; ++a) {
+    const int i = i0 + wr * (BM / WM) + a * 16 + li;
     if (i >= m) continue;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < TN; ++b) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = j0 + wc * 64 + b * 16 + lk + 4 * r;
+        const int j = j0 + wc * (BN / WN) + b * 16 + lk + 4 * r;
         if (j >= n) continue;
         if (mode == 1 && i < j) continue;
         double* c = C + (long)i + (long)j * ldc;
@@ -193,36 +177,83 @@ __global__ __launch_bounds__(NTHR) void dgemm_mfma_kernel(
   }
 }
 
+// Tile configurations (selected by fla_hip_set_gemm_config for tuning).
+int g_gemm_cfg = 0;
+
+template <int BM, int BN, int BK, int WM, int WN>
+int launch_cfg(hipStream_t s, bool ta, bool tb, int mode, int m, int n, int k, double alpha,
+               const double* A, long lda, const double* B, long ldb, double beta, double* C,
+               long ldc, const int* abort_flag) {
+  static_assert(BM == BN, "the lower-tile enumeration assumes square tiles");
+  const int Tm = (m + BM - 1) / BM, Tn = (n + BN - 1) / BN;
+  const int rpc_eff = 1;  // column tile c starts at row tile c
+  const long nblk = (mode == 0) ? (long)Tm * Tn : (long)Tn * Tm - (long)Tn * (Tn - 1) / 2;
+  if (nblk <= 0) return FLA_HIP_OK;
+  if (nblk > 0x7fffffffL) return FLA_HIP_EARG;
+  dim3 grid((unsigned)nblk), block(64 * WM * WN);
+#define FLA_GEMM_LAUNCH(TA_, TB_)                                                              \
+  hipLaunchKernelGGL((dgemm_mfma_kernel<TA_, TB_, BM, BN, BK, WM, WN>), grid, block, 0, s,     \
+                     mode, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, Tm, Tn, rpc_eff,       \
+                     abort_flag)
+  if (!ta && !tb)
+    FLA_GEMM_LAUNCH(false, false);
+  else if (!ta && tb)
+    FLA_GEMM_LAUNCH(false, true);
+  else if (ta && !tb)
+    FLA_GEMM_LAUNCH(true, false);
+  else
+    FLA_GEMM_LAUNCH(true, true);
+#undef FLA_GEMM_LAUNCH
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
 }  // namespace
+
+extern "C" int fla_hip_set_gemm_config(int cfg) {
+  if (cfg < 0 || cfg > 7) return FLA_HIP_EARG;
+  g_gemm_cfg = cfg;
+  return FLA_HIP_OK;
+}
 
 int launch_dgemm_mfma(hipStream_t s, bool ta, bool tb, int mode, int m, int n, int k,
                       double alpha, const double* A, long lda, const double* B, long ldb,
                       double beta, double* C, long ldc, const int* abort_flag) {
   if (m <= 0 || n <= 0) return FLA_HIP_OK;
   if (mode == 1 && m < n) return FLA_HIP_EARG;  // lower mode expects a tall C
-  const int Tm = (m + BM - 1) / BM, Tn = (n + BN - 1) / BN;
-  long nblk;
-  if (mode == 0)
-    nblk = (long)Tm * Tn;
-  else
-    nblk = (long)Tn * Tm - (long)Tn * (Tn - 1) / 2;
-  if (nblk <= 0) return FLA_HIP_OK;
-  if (nblk > 0x7fffffffL) return FLA_HIP_EARG;
-  dim3 grid((unsigned)nblk), block(NTHR);
-  if (!ta && !tb)
-    hipLaunchKernelGGL((dgemm_mfma_kernel<false, false>), grid, block, 0, s, mode, m, n, k,
-                       alpha, A, lda, B, ldb, beta, C, ldc, Tm, Tn, abort_flag);
-  else if (!ta && tb)
-    hipLaunchKernelGGL((dgemm_mfma_kernel<false, true>), grid, block, 0, s, mode, m, n, k,
-                       alpha, A, lda, B, ldb, beta, C, ldc, Tm, Tn, abort_flag);
-  else if (ta && !tb)
-    hipLaunchKernelGGL((dgemm_mfma_kernel<true, false>), grid, block, 0, s, mode, m, n, k,
-                       alpha, A, lda, B, ldb, beta, C, ldc, Tm, Tn, abort_flag);
-  else
-    hipLaunchKernelGGL((dgemm_mfma_kernel<true, true>), grid, block, 0, s, mode, m, n, k,
-                       alpha, A, lda, B, ldb, beta, C, ldc, Tm, Tn, abort_flag);
-  FLA_HIP_CHECK(hipGetLastError());
-  return FLA_HIP_OK;
+  switch (g_gemm_cfg) {
+    case 1:
+      return launch_cfg<128, 128, 32, 2, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+    case 2:
+      return launch_cfg<128, 128, 16, 4, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+    case 3:
+      return launch_cfg<128, 128, 32, 4, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+    case 4:
+      return launch_cfg<128, 128, 16, 2, 4>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+    case 5:
+      return launch_cfg<128, 128, 16, 4, 4>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+    case 6:
+      return launch_cfg<128, 128, 8, 4, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                           C, ldc, abort_flag);
+    case 7:
+      return launch_cfg<128, 128, 16, 2, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+    default:
+      // Measured on MI355X (n = 16384..32768, k = 128..256): 8 waves per
+      // 128x128 tile; K chunks of 16 when B is stored with j contiguous (the
+      // SYRK update, ~52 TFLOP/s), of 8 when B is k-contiguous (the LU
+      // update, ~39 TFLOP/s), where the smaller LDS footprint doubles occupancy.
+      if (!tb)
+        return launch_cfg<128, 128, 8, 4, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb,
+                                             beta, C, ldc, abort_flag);
+      return launch_cfg<128, 128, 16, 4, 2>(s, ta, tb, mode, m, n, k, alpha, A, lda, B, ldb, beta,
+                                            C, ldc, abort_flag);
+  }
 }
 
 static bool is_trans(int t) { return t == kTrans || t == kConjTrans; }

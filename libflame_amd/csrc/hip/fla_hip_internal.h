@@ -38,6 +38,8 @@ struct FlaHipCtx {
   size_t work_bytes = 0;
   // Small persistent device scratch: [0] info word, [1..] counters.
   int* dscratch = nullptr;
+  // 128 x 128 FP64 block: inverse of the current Cholesky diagonal block.
+  double* dlinv = nullptr;
   int chol_nb = 256;
   int lu_nb = 128;
 };

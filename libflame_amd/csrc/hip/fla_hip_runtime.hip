@@ -40,6 +40,7 @@ extern "C" int fla_hip_init(void) {
   for (int i = 0; i < FlaHipCtx::kEvents; ++i)
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
   FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dscratch), 4096));
+  FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dlinv), 128 * 128 * sizeof(double)));
   g_ctx.ready = true;
   return FLA_HIP_OK;
 }
@@ -52,6 +53,8 @@ extern "C" int fla_hip_finalize(void) {
   g_ctx.work_bytes = 0;
   if (g_ctx.dscratch) FLA_HIP_CHECK(hipFree(g_ctx.dscratch));
   g_ctx.dscratch = nullptr;
+  if (g_ctx.dlinv) FLA_HIP_CHECK(hipFree(g_ctx.dlinv));
+  g_ctx.dlinv = nullptr;
   for (int i = 0; i < FlaHipCtx::kEvents; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev[i]));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_panel));
