@@ -125,6 +125,137 @@ __global__ __launch_bounds__(1024) void potrf_tile_kernel(int nb, double* __rest
   if (t == 0 && fail >= 0) *info = col_offset + fail;
 }
 
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+constexpr int DLD = PB + 1;  // LDS leading dimension of the diagonal block
+
+// Single-workgroup Cholesky of the nb x nb lower block at A (nb <= 128),
+// blocked by 32 inside the workgroup (the reference's var3 structure one
+// level down, FLA_Chol_l_blk_var3.c:13-66):
+//   1. wave 0 factors the 32x32 diagonal block, lane = row, pivots broadcast
+//      with readlane (no barriers): d = sqrt(a_jj) with bl1_dsqrte's failure
+//      rule, l_ij = a_ij * (1/d), a_ic -= l_ij l_cj;
+//   2. the rows below are solved against it (lane = row, row in VGPRs, the
+//      32x32 factor read from LDS by broadcast), dtrsm's update order;
+//   3. the trailing part is updated with 4x4 register tiles (rank 32).
+// The block lives in LDS (129 KB); padding beyond nb is the identity.
+__global__ __launch_bounds__(256) void potrf_blk_kernel(int nb, double* __restrict__ A, long lda,
+                                                        int col_offset, int* __restrict__ info) {
+  if (*info >= 0) return;
+  __shared__ double S[PB * DLD];
+  __shared__ int fail;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  for (int e = t; e < PB * PB; e += 256) {
+    const int i = e & (PB - 1), c = e >> 7;
+    double v = 0.0;
+    if (i >= c) {
+      if (i < nb && c < nb)
+        v = A[(long)i + (long)c * lda];
+      else if (i == c)
+        v = 1.0;
+    }
+    S[c * DLD + i] = v;
+  }
+  if (t == 0) fail = -1;
+  __syncthreads();
+  for (int b = 0; b < PB; b += LB) {
+    // 1. diagonal 32x32 block: wave 0, lanes 0..31
+    if (wave == 0) {
+      const int r = b + (lane & 31);
+      double x[LB];
+#pragma unroll
+      for (int c = 0; c < LB; ++c) x[c] = S[(b + c) * DLD + r];
+      int f = -1;
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        if (f >= 0) continue;  // uniform: a failed column stops the sweep
+        const double ajj = readlane_d(x[j], j);
+        if (!(ajj > 0.0)) {  // <= 0 or NaN, as bl1_dsqrte
+          f = j;
+          continue;
+        }
+        const double d = sqrt(ajj);
+        const double inv = 1.0 / d;
+        if ((lane & 31) > j) x[j] *= inv;
+        if ((lane & 31) == j) x[j] = d;
+#pragma unroll
+        for (int c = j + 1; c < LB; ++c) {
+          const double lcj = readlane_d(x[j], c);
+          if ((lane & 31) >= c) x[c] -= x[j] * lcj;
+        }
+      }
+      if (lane < 32) {
+#pragma unroll
+        for (int c = 0; c < LB; ++c)
+          if (lane >= c && (f < 0 || c < f)) S[(b + c) * DLD + r] = x[c];
+      }
+      if (lane == 0 && f >= 0) fail = b + f;
+    }
+    __syncthreads();
+    if (fail >= 0) break;
+    // 2. rows below the diagonal block: X := X * inv(L_bb)'
+    const int nrow = PB - b - LB;
+    if (t < nrow) {
+      const int r = b + LB + t;
+      double x[LB];
+#pragma unroll
+      for (int c = 0; c < LB; ++c) x[c] = S[(b + c) * DLD + r];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        x[j] *= 1.0 / S[(b + j) * DLD + b + j];
+#pragma unroll
+        for (int c = j + 1; c < LB; ++c) x[c] -= S[(b + j) * DLD + b + c] * x[j];
+      }
+#pragma unroll
+      for (int c = 0; c < LB; ++c) S[(b + c) * DLD + r] = x[c];
+    }
+    __syncthreads();
+    // 3. trailing update: A(i, c) -= sum_k L(i, k) L(c, k), i >= c >= b + 32
+    const int T4 = nrow / 4;
+    const int ntile = T4 * (T4 + 1) / 2;
+    for (int tt = t; tt < ntile; tt += 256) {
+      int I = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
+      while (I * (I + 1) / 2 > tt) --I;
+      while ((I + 1) * (I + 2) / 2 <= tt) ++I;
+      const int J = tt - I * (I + 1) / 2;
+      const int i0 = b + LB + 4 * I, c0 = b + LB + 4 * J;
+      double acc[4][4] = {};
+#pragma unroll 8
+      for (int k = 0; k < LB; ++k) {
+        const double* col = S + (b + k) * DLD;
+        double li[4], lc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          li[q] = col[i0 + q];
+          lc[q] = col[c0 + q];
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) acc[x][y] += li[x] * lc[y];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          if (i0 + x >= c0 + y) S[(c0 + y) * DLD + i0 + x] -= acc[x][y];
+    }
+    __syncthreads();
+  }
+  const int ncols = (fail >= 0) ? fail : nb;
+  for (int e = t; e < PB * PB; e += 256) {
+    const int i = e & (PB - 1), c = e >> 7;
+    if (i >= c && i < nb && c < ncols) A[(long)i + (long)c * lda] = S[c * DLD + i];
+  }
+  if (t == 0 && fail >= 0) *info = col_offset + fail;
+}
+
 // Leaf of the panel solve: X := X * inv(L)' for an m x nb strip (nb <= 32),
 // L lower non-unit.  One lane per row with the row in VGPRs, L through
 // uniform loads; the update order of dtrsm_('R','L','T','N')
@@ -189,7 +320,7 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 int launch_potrf_diag(hipStream_t s, int nb, double* A, long lda, int col_offset, int* info) {
   if (nb <= 0) return FLA_HIP_OK;
   if (nb > PB) return FLA_HIP_EARG;
-  hipLaunchKernelGGL(potrf_tile_kernel, dim3(1), dim3(1024), 0, s, nb, A, lda, col_offset, info);
+  hipLaunchKernelGGL(potrf_blk_kernel, dim3(1), dim3(256), 0, s, nb, A, lda, col_offset, info);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

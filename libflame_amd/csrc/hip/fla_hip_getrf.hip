@@ -39,7 +39,7 @@ struct PanelComm {
   double* hdr;      // [2][P][HDR]
   double* rows;     // [2][P][3][PW]   best row, NaN row, after-NaN row
   double* jrow;     // [2][PW + 1]     diagonal row values (+ orig)
-  unsigned* bar;    // [0] arrivals, [1] generation
+  int* seq;         // [2][P]          column (+1) whose slot is published
   int* err;         // wait-limit flag
 };
 
@@ -48,38 +48,6 @@ __device__ __forceinline__ void st_d(double* p, double v) {
 }
 __device__ __forceinline__ double ld_d(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid-wide barrier number `gen` (0, 1, 2, ...) among P workgroups.  Every
-// wave fences its own agent-scope stores before the workgroup barrier; one
-// lane per workgroup arrives and waits (bounded by the wall clock).
-__device__ bool grid_barrier(unsigned* bar, int* err, unsigned P, unsigned gen) {
-  __shared__ int ok;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ok = 1;
-    const unsigned target = P * (gen + 1);
-    const unsigned prev =
-        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev + 1 == target) {
-      __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const unsigned long long t0 = wall_clock64();
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < gen + 1) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-            wall_clock64() - t0 > kSpinTicks) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ok != 0;
 }
 
 // First-max combine of (value, row): larger value wins, ties -> smaller row.
@@ -121,7 +89,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
   __shared__ double cand[3][PW];
   __shared__ double wred[2][8];
   __shared__ double hsum[8];
-  __shared__ int sel[4];        // winner row, winner orig, winner workgroup, slot
+  __shared__ int sel[4];        // winner row, winner orig, orig of the old diagonal row
+  __shared__ int jorig;
 
   // Thread owns rows base + rh*128 + 64*q + lane (q = 0..1), columns 32*cg + c.
   double v[RQ][32];
@@ -137,7 +106,6 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     }
   }
 
-  unsigned gen = 0;
   for (int j = 0; j < W; ++j) {
     const int par = j & 1;
     const int cgj = j >> 5, cj = j & 31;
@@ -238,7 +206,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     }
     __syncthreads();
     const int br = (int)hsum[1], nr = (int)hsum[3], ar = (int)hsum[6];
-    // Stage candidate rows (and the diagonal row j) in LDS, then publish.
+    // Stage candidate rows (and the diagonal row j) in LDS.
 #pragma unroll
     for (int q = 0; q < RQ; ++q) {
       const int r = base + rh * 128 + 64 * q + lane;
@@ -250,104 +218,131 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         if (r == ar) cand[2][col] = v[q][c];
         if (r == j) jrow[col] = v[q][c];
       }
-      if (r == j && cg == 0) st_d(&jslot[PW], (double)orig[q]);
+      if (r == j && cg == 0) jorig = orig[q];
     }
     __syncthreads();
-    if (t < HDR) st_d(&hdr[t], hsum[t]);
-    if (t < W) {
-      if (br >= 0) st_d(&rows[0 * PW + t], cand[0][t]);
-      if (nr >= 0) st_d(&rows[1 * PW + t], cand[1][t]);
-      if (ar >= 0) st_d(&rows[2 * PW + t], cand[2][t]);
-      if (j >= base && j < base + ROWS_PER_WG) st_d(&jslot[t], jrow[t]);
-    }
-    if (!grid_barrier(cm.bar, cm.err, P, gen++)) return;
-
-    // ---- global winner (wave 0 combines the P headers) ----
-    if (t < 64) {
-      double gnr = -1.0, gno = -1.0;
-      for (unsigned w = t; w < P; w += 64) {
-        const double* h = cm.hdr + ((long)par * P + w) * HDR;
-        const double hn = ld_d(&h[3]);
-        if (hn > gnr) {
-          gnr = hn;
-          gno = ld_d(&h[4]);
-        }
+    // ---- wave 0: publish this workgroup's slot for column j, wait for every
+    // slot of column j, then pick the global pivot ----
+    if (wave == 0) {
+      const bool owns_j = (j >= base && j < base + ROWS_PER_WG);
+      if (lane < HDR) st_d(&hdr[lane], hsum[lane]);
+      for (int e = lane; e < W; e += 64) {
+        if (br >= 0) st_d(&rows[0 * PW + e], cand[0][e]);
+        if (nr >= 0) st_d(&rows[1 * PW + e], cand[1][e]);
+        if (ar >= 0) st_d(&rows[2 * PW + e], cand[2][e]);
+        if (owns_j) st_d(&jslot[e], jrow[e]);
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(gnr, off), oo = __shfl_xor(gno, off);
-        if (o > gnr) {
-          gnr = o;
-          gno = oo;
+      if (owns_j && lane == 0) st_d(&jslot[PW], (double)jorig);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (lane == 0)
+        __hip_atomic_store(&cm.seq[par * P + wg], j + 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      bool ok = true;
+      const unsigned long long t0 = wall_clock64();
+      while (true) {
+        bool done = true;
+        for (unsigned w = lane; w < P; w += 64)
+          if (__hip_atomic_load(&cm.seq[par * P + w], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT) != j + 1)
+            done = false;
+        if (__all(done)) break;
+        if (__hip_atomic_load(cm.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            wall_clock64() - t0 > kSpinTicks) {
+          if (lane == 0) __hip_atomic_store(cm.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = false;
+          break;
         }
+        __builtin_amdgcn_s_sleep(1);
       }
-      // Candidates: without a NaN, every slab's best; with the last NaN at
-      // row gnr (idamax then restarts after it), the after-NaN candidate of
-      // the slab holding gnr plus the best of every slab wholly below it.
-      double bv = 0.0;
-      int bw = -1, brow = -1, bo = -1, bslot = 0;
-      for (unsigned w = t; w < P; w += 64) {
-        const double* h = cm.hdr + ((long)par * P + w) * HDR;
-        double cv = 0.0;
-        int cr = -1, co = -1, cs = 0;
-        if (gnr < 0.0) {
-          cv = ld_d(&h[0]);
-          cr = (int)ld_d(&h[1]);
-          co = (int)ld_d(&h[2]);
-        } else {
-          const int hn = (int)ld_d(&h[3]);
-          if (hn == (int)gnr) {
-            cv = ld_d(&h[5]);
-            cr = (int)ld_d(&h[6]);
-            co = (int)ld_d(&h[7]);
-            cs = 2;
-          } else if ((long)w * ROWS_PER_WG > (long)gnr) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (!ok) {
+        if (lane == 0) sel[0] = -2;
+      } else {
+        double gnr = -1.0, gno = -1.0;
+        for (unsigned w = lane; w < P; w += 64) {
+          const double* h = cm.hdr + ((long)par * P + w) * HDR;
+          const double hn = ld_d(&h[3]);
+          if (hn > gnr) {
+            gnr = hn;
+            gno = ld_d(&h[4]);
+          }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          const double o = __shfl_xor(gnr, off), oo = __shfl_xor(gno, off);
+          if (o > gnr) {
+            gnr = o;
+            gno = oo;
+          }
+        }
+        // Candidates: without a NaN, every slab's best; with the last NaN at
+        // row gnr (idamax then restarts after it), the after-NaN candidate of
+        // the slab holding gnr plus the best of every slab wholly below it.
+        double bv = 0.0;
+        int bw = -1, brow = -1, bo = -1, bslot = 0;
+        for (unsigned w = lane; w < P; w += 64) {
+          const double* h = cm.hdr + ((long)par * P + w) * HDR;
+          double cv = 0.0;
+          int cr = -1, co = -1, cs = 0;
+          if (gnr < 0.0) {
             cv = ld_d(&h[0]);
             cr = (int)ld_d(&h[1]);
             co = (int)ld_d(&h[2]);
+          } else {
+            const int hn = (int)ld_d(&h[3]);
+            if (hn == (int)gnr) {
+              cv = ld_d(&h[5]);
+              cr = (int)ld_d(&h[6]);
+              co = (int)ld_d(&h[7]);
+              cs = 2;
+            } else if ((long)w * ROWS_PER_WG > (long)gnr) {
+              cv = ld_d(&h[0]);
+              cr = (int)ld_d(&h[1]);
+              co = (int)ld_d(&h[2]);
+            }
+          }
+          if (cr >= 0 && (brow < 0 || cv > bv || (cv == bv && cr < brow))) {
+            bv = cv;
+            brow = cr;
+            bo = co;
+            bw = (int)w;
+            bslot = cs;
           }
         }
-        if (cr >= 0 && (brow < 0 || cv > bv || (cv == bv && cr < brow))) {
-          bv = cv;
-          brow = cr;
-          bo = co;
-          bw = (int)w;
-          bslot = cs;
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(bv, off);
+          const int orow = __shfl_xor(brow, off), oo = __shfl_xor(bo, off),
+                    ow = __shfl_xor(bw, off), os = __shfl_xor(bslot, off);
+          if (orow >= 0 && (brow < 0 || ov > bv || (ov == bv && orow < brow))) {
+            bv = ov;
+            brow = orow;
+            bo = oo;
+            bw = ow;
+            bslot = os;
+          }
         }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int orow = __shfl_xor(brow, off), oo = __shfl_xor(bo, off),
-                  ow = __shfl_xor(bw, off), os = __shfl_xor(bslot, off);
-        if (orow >= 0 && (brow < 0 || ov > bv || (ov == bv && orow < brow))) {
-          bv = ov;
-          brow = orow;
-          bo = oo;
-          bw = ow;
-          bslot = os;
+        if (brow < 0 && gnr >= 0.0) {  // nothing after the last NaN: it is the pivot
+          brow = (int)gnr;
+          bo = (int)gno;
+          bw = brow / ROWS_PER_WG;
+          bslot = 1;
         }
-      }
-      if (brow < 0 && gnr >= 0.0) {  // nothing after the last NaN: it is the pivot
-        brow = (int)gnr;
-        bo = (int)gno;
-        bw = brow / ROWS_PER_WG;
-        bslot = 1;
-      }
-      if (t == 0) {
-        sel[0] = brow;
-        sel[1] = bo;
-        sel[2] = bw;
-        sel[3] = bslot;
+        // the winning row and the old diagonal row (from its owner's slot)
+        const double* src = cm.rows + (((long)par * P + bw) * 3 + bslot) * PW;
+        for (int e = lane; e < W; e += 64) {
+          prow[e] = ld_d(&src[e]);
+          jrow[e] = ld_d(&jslot[e]);
+        }
+        if (lane == 0) {
+          sel[0] = brow;
+          sel[1] = bo;
+          sel[2] = (int)ld_d(&jslot[PW]);
+        }
       }
     }
     __syncthreads();
+    if (sel[0] == -2) return;
     const int r = sel[0];
-    if (t < W) {
-      const double* src = cm.rows + (((long)par * P + sel[2]) * 3 + sel[3]) * PW;
-      prow[t] = ld_d(&src[t]);
-      jrow[t] = ld_d(&jslot[t]);
-    }
-    const int origj = (int)ld_d(&jslot[PW]);
-    __syncthreads();
+    const int origj = sel[2];
 
     const double piv_val = prow[j];
     const bool null_pivot = (piv_val == 0.0);
@@ -460,10 +455,9 @@ __global__ __launch_bounds__(256) void lu_row_gather_kernel(int ncols, double* _
   }
 }
 
-__global__ void lu_reset_kernel(unsigned* bar, int* moved) {
-  bar[0] = 0;
-  bar[1] = 0;
-  moved[0] = 0;
+__global__ void lu_reset_kernel(int* seq, int nseq, int* moved) {
+  for (int i = threadIdx.x; i < nseq; i += blockDim.x) seq[i] = 0;
+  if (threadIdx.x == 0) moved[0] = 0;
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
@@ -498,7 +492,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static size_t lu_set_bytes(int Pmax) {
   return align256(sizeof(double) * 2 * Pmax * HDR) + align256(sizeof(double) * 2 * Pmax * 3 * PW) +
-         align256(sizeof(double) * 2 * (PW + 1)) + align256(2 * sizeof(unsigned)) +
+         align256(sizeof(double) * 2 * (PW + 1)) + align256(sizeof(int) * 2 * Pmax) +
          align256(sizeof(int)) + align256(sizeof(int) * (1 + 4 * PW));
 }
 
@@ -510,8 +504,8 @@ static LuBufs lu_carve(char* p, int Pmax) {
   p += align256(sizeof(double) * 2 * Pmax * 3 * PW);
   b.cm.jrow = reinterpret_cast<double*>(p);
   p += align256(sizeof(double) * 2 * (PW + 1));
-  b.cm.bar = reinterpret_cast<unsigned*>(p);
-  p += align256(2 * sizeof(unsigned));
+  b.cm.seq = reinterpret_cast<int*>(p);
+  p += align256(sizeof(int) * 2 * Pmax);
   b.cm.err = reinterpret_cast<int*>(p);
   p += align256(sizeof(int));
   b.moved = reinterpret_cast<int*>(p);
@@ -520,7 +514,7 @@ static LuBufs lu_carve(char* p, int Pmax) {
 
 static int lu_panel(hipStream_t s, int M, int W, double* Ap, long lda, int k, int* piv,
                     int* dinfo, const LuBufs& b) {
-  hipLaunchKernelGGL(lu_reset_kernel, dim3(1), dim3(1), 0, s, b.cm.bar, b.moved);
+  hipLaunchKernelGGL(lu_reset_kernel, dim3(1), dim3(256), 0, s, b.cm.seq, 2 * ((M + ROWS_PER_WG - 1) / ROWS_PER_WG), b.moved);
   const int P = (M + ROWS_PER_WG - 1) / ROWS_PER_WG;
   void* tok = fla_stats_begin(s);
   hipLaunchKernelGGL(lu_panel_kernel, dim3(P), dim3(NT), 0, s, M, W, Ap, lda, k, piv, dinfo,
