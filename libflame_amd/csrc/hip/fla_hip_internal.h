@@ -91,5 +91,13 @@ int launch_trsm_rlt(hipStream_t s, int m, int nb, const double* L11, long ldl, d
 // A12 := inv(L11) * A12, L11 unit lower nb x nb (nb <= 128): one lane per column.
 int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long ldl, double* A12,
                      long lda);
+// X := inv(op(T)) X for na x nrhs X (left side), W = inverted 128x128 diagonal
+// blocks of T in T's orientation; one persistent launch per 16 columns.
+// done/err are two device ints the call resets; *err != 0 after a bounded
+// wait expired.
+constexpr int kNarrowRhs = 32;
+int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, int nrhs,
+                            const double* T, long ldt, const double* W, double* X, long ldx,
+                            int* done, int* err);
 // In-place exchange of the strictly lower and strictly upper triangles.
 int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

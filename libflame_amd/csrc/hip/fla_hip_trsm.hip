@@ -20,17 +20,60 @@ namespace {
 constexpr int TB = 128;       // diagonal block order
 constexpr int TLD = TB + 1;   // LDS leading dimension
 
+// One level of the recursive inversion in LDS: for every 2H-block at offset o,
+// the lower-left H x H block B becomes -inv(C) * B * inv(A) (A, C already
+// inverted).  Each of the 256 threads holds H/4 outputs in registers.
+template <int H>
+__device__ __forceinline__ void trtri_combine(double* S, int t) {
+  constexpr int PER = H / 4;
+  double reg[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = t + 256 * q;
+    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
+    double sum = 0.0;
+    for (int kk = j; kk < H; ++kk) sum += S[(o + kk) * TLD + o + H + i] * S[(o + j) * TLD + o + kk];
+    reg[q] = sum;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = t + 256 * q;
+    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
+    S[(o + j) * TLD + o + H + i] = reg[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = t + 256 * q;
+    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
+    double sum = 0.0;
+    for (int kk = 0; kk <= i; ++kk) sum += S[(o + H + kk) * TLD + o + H + i] * S[(o + j) * TLD + o + H + kk];
+    reg[q] = -sum;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = t + 256 * q;
+    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
+    S[(o + j) * TLD + o + H + i] = reg[q];
+  }
+  __syncthreads();
+}
+
 // Inverts the nbk x nbk diagonal block k of a triangular matrix (lower or
 // upper storage, unit or non-unit diagonal) into W (ld TB per block, same
 // triangle orientation as A; the other triangle of W is zeroed).
-// The block is loaded as a lower triangle L (transposed for upper storage),
-// inverted in place column by column from the right (dtrti2's order):
-//   L(j,j) := 1/L(j,j);  L(j+1:,j) := -L(j,j) * inv(L)(j+1:,j+1:) * L(j+1:,j)
+// The block is loaded as a lower triangle L (transposed for upper storage)
+// and inverted in LDS by recursive 2x2 blocking,
+//   inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A)  inv(C)],
+// with the four 32x32 diagonal blocks solved by one wave each (lane = column
+// of the inverse, forward substitution, L read by broadcast) and the
+// off-diagonal blocks at sizes 64 and 128 formed as register-held products.
 __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* __restrict__ A,
                                                            long lda, int upper, int unit,
                                                            double* __restrict__ W) {
   __shared__ double S[TB * TLD];
-  __shared__ double tmp[TB];
   const int k = blockIdx.x;
   const int r0 = k * TB;
   const int nbk = (n - r0) < TB ? (n - r0) : TB;
@@ -46,19 +89,30 @@ __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* 
     S[j * TLD + i] = v;
   }
   __syncthreads();
-  for (int j = TB - 1; j >= 0; --j) {
-    const double ajj = 1.0 / S[j * TLD + j];
-    if (t > j && t < TB) tmp[t] = S[j * TLD + t];
-    __syncthreads();
-    if (t > j && t < TB) {
-      // y_t = sum_{c=j+1}^{t} invL(t,c) * L(c,j)
-      double y = 0.0;
-      for (int c = j + 1; c <= t; ++c) y += S[c * TLD + t] * tmp[c];
-      S[j * TLD + t] = -ajj * y;
+  {
+    const int w = t >> 6, lane = t & 63;
+    if (lane < 32) {
+      const int o = 32 * w, c = lane;
+      double y[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        double sum = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int kk = 0; kk < i; ++kk) sum -= S[(o + kk) * TLD + o + i] * y[kk];
+        y[i] = sum / S[(o + i) * TLD + o + i];
+        // Pins y[i] (and the LDS reads) to this iteration: otherwise the
+        // compiler hoists every broadcast read of later rows and spills.
+        asm volatile("" : "+v"(y[i])::"memory");
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int i = 0; i < 32; ++i)
+        if (i >= c) S[(o + c) * TLD + o + i] = y[i];
     }
-    if (t == 0) S[j * TLD + j] = ajj;
-    __syncthreads();
   }
+  __syncthreads();
+  trtri_combine<32>(S, t);
+  trtri_combine<64>(S, t);
   double* Wk = W + (long)k * TB * TB;
   for (int idx = t; idx < TB * TB; idx += 256) {
     const int i = idx % TB, j = idx / TB;  // W(i,j), ld TB
@@ -154,6 +208,14 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
   FLA_HIP_CHECK(hipGetLastError());
   // op(A) is lower when (lower, N) or (upper, T): forward substitution.
   const bool op_lower = (!upper && !tr) || (upper && tr);
+  if (side == kLeft && n <= kNarrowRhs) {
+    int* flags = reinterpret_cast<int*>(W + (long)nblk * TB * TB);
+    FLA_TRY(launch_trsm_left_narrow(s, op_lower, tr, na, n, A, lda, W, B, ldb, flags, flags + 1));
+    int herr = 0;
+    FLA_HIP_CHECK(hipMemcpyAsync(&herr, flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    FLA_HIP_CHECK(hipStreamSynchronize(s));
+    return herr ? FLA_HIP_ESPIN : FLA_HIP_OK;
+  }
   auto blk = [&](int idx) { return idx * TB; };
   if (side == kLeft) {
     // op(A) X = B.  Forward when op(A) is lower.

@@ -56,12 +56,14 @@ def test_dsyrk_exact(orc, uplo, trans):
 @pytest.mark.parametrize("uplo", [L, U])
 @pytest.mark.parametrize("trans", [N, T])
 @pytest.mark.parametrize("diag", [fla.FLA_NONUNIT_DIAG, fla.FLA_UNIT_DIAG])
-def test_dtrsm_all_variants_vs_oracle(orc, side, uplo, trans, diag):
+@pytest.mark.parametrize("dim,rhs", [(300, 1), (300, 21), (300, 40), (2000, 16)])
+def test_dtrsm_all_variants_vs_oracle(orc, side, uplo, trans, diag, dim, rhs):
     """Well-conditioned triangles (the GPU solve multiplies by inverted
     128x128 diagonal blocks, so exactness of dyadic data is not preserved):
-    agreement with the oracle's dtrsm_ to 1e-12 and a backward error <= 10."""
-    m, n = (300, 21) if side == fla.FLA_LEFT else (21, 300)
-    dim = m if side == fla.FLA_LEFT else n
+    agreement with the oracle's dtrsm_ to 1e-12 and a backward error <= 10.
+    Left solves with <= 32 right-hand sides take the persistent block-row
+    kernel (16 columns per launch), wider ones the MFMA block substitution."""
+    m, n = (dim, rhs) if side == fla.FLA_LEFT else (rhs, dim)
     rng = np.random.default_rng(23)
     Tm = rng.uniform(-1, 1, (dim, dim)) / np.sqrt(dim)
     Tm = np.tril(Tm) if uplo == L else np.triu(Tm)
