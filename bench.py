@@ -87,6 +87,211 @@ def cpu_baseline(n_cpu):
                       f"n={n_cpu}, A=B*B'/n+I, {dt:.1f} s on one host core"}
 
 
+def bench_mixed(fla, np, n, nrhs=8):
+    """Mixed-precision solve (fla_hip_dsposv) against the plain FP64 solve
+    (FLA_Chol + two triangular solves) on A = Q diag(s) Q^T, Q a product of
+    64 Householder reflectors, s log-spaced on [1, kappa]; 8 N(0,1) RHS.
+    Time-to-solve of each, normwise backward error max_j ||r_j||/(||A|| ||x_j||)
+    (infinity norms) and forward error against the FP64 solution."""
+    import ctypes as C
+    lib = fla.lib()
+    res = {"n": n, "nrhs": nrhs, "cases": []}
+    A0 = fla.DeviceMatrix(n, n)
+    A = fla.DeviceMatrix(n, n)
+    B = fla.DeviceMatrix(n, nrhs)
+    X = fla.DeviceMatrix(n, nrhs)
+    X64 = fla.DeviceMatrix(n, nrhs)
+    R = fla.DeviceMatrix(n, nrhs)
+    fla.check(lib.fla_hip_gen_normal(5, n, nrhs, B.ptr, n))
+    for kappa in (1e2, 1e4, 1e6):
+        log(f"mixed: kappa={kappa:g}, n={n}")
+        fla.check(lib.fla_hip_gen_spd_reflectors(11, n, 64, kappa, A0.ptr, n))
+        fla.synchronize()
+        # plain FP64: FLA_Chol + two triangular solves
+        t64 = []
+        for rep in range(2):
+            copy(fla, A, A0)
+            copy(fla, X64, B)
+            fla.synchronize()
+            t0 = time.perf_counter()
+            assert fla.FLA_Chol(fla.FLA_LOWER_TRIANGULAR, A.obj()) == fla.FLA_SUCCESS
+            for tr in (fla.FLA_NO_TRANSPOSE, fla.FLA_TRANSPOSE):
+                fla.check(lib.fla_hip_dtrsm(fla.FLA_LEFT, fla.FLA_LOWER_TRIANGULAR, tr,
+                                            fla.FLA_NONUNIT_DIAG, n, nrhs, 1.0, A.ptr, n,
+                                            X64.ptr, n))
+            fla.synchronize()
+            t64.append(time.perf_counter() - t0)
+        ms64 = min(t64) * 1e3
+        bwd64 = backward_error(fla, np, A0, B, X64, R, n, nrhs)
+        case = {"kappa": kappa, "fp64_ms": round(ms64, 2), "fp64_bwd_err_over_eps": bwd64}
+        for name, prec, ref in (("fp32_classic", fla.FLA_MP_FP32, fla.FLA_IR_CLASSIC),
+                                ("fp32_gmres", fla.FLA_MP_FP32, fla.FLA_IR_GMRES),
+                                ("bf16_gmres", fla.FLA_MP_BF16, fla.FLA_IR_GMRES),
+                                ("bf16_classic", fla.FLA_MP_BF16, fla.FLA_IR_CLASSIC)):
+            ts = []
+            for rep in range(2):
+                copy(fla, A, A0)
+                fla.synchronize()
+                t0 = time.perf_counter()
+                it, info, st = fla.dsposv(fla.FLA_LOWER_TRIANGULAR, A, B, X, precision=prec,
+                                          refinement=ref)
+                fla.synchronize()
+                ts.append(time.perf_counter() - t0)
+            bwd = backward_error(fla, np, A0, B, X, R, n, nrhs)
+            xh, x64 = X.download(), X64.download()
+            fwd = float(np.max(np.abs(xh - x64)) / np.max(np.abs(x64)))
+            case[name] = {"ms": round(min(ts) * 1e3, 2), "iter": it, "info": info,
+                          "bwd_err_over_eps": bwd, "fwd_err_vs_fp64": fwd,
+                          "factor_ms": round(st["factor_ms"], 2),
+                          "refine_ms": round(st["refine_ms"], 2),
+                          "prec_solves": int(st["prec_solves"]),
+                          "gmres_iters": int(st["gmres_iters"]),
+                          "time_ratio_vs_fp64": round(min(ts) * 1e3 / ms64, 3)}
+            log(f"  {name}: {case[name]}")
+        res["cases"].append(case)
+    del A0, A, B, X, X64, R
+    return res
+
+
+def backward_error(fla, np, A0, B, X, R, n, nrhs):
+    """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) / eps, with the
+    product computed in FP64 on the GPU from the full symmetric A."""
+    fla.check(fla.lib().fla_hip_dmatmul_check(1, fla.FLA_LOWER_TRIANGULAR, fla.FLA_NO_TRANSPOSE,
+                                               fla.FLA_NONUNIT_DIAG, n, nrhs, A0.ptr, n, X.ptr,
+                                               n, 0.0, R.ptr, n))
+    fla.synchronize()
+    AX, Bh, Xh = R.download(), B.download(), X.download()
+    anorm = _sym_inf_norm(fla, np, A0, n)
+    e = 0.0
+    for j in range(nrhs):
+        e = max(e, np.max(np.abs(Bh[:, j] - AX[:, j])) / (anorm * np.max(np.abs(Xh[:, j]))))
+    return float(e / EPS)
+
+
+def _sym_inf_norm(fla, np, A0, n):
+    """||A||_inf of the symmetric A from its full storage, block rows on the host."""
+    best = 0.0
+    step = 2048
+    for r0 in range(0, n, step):
+        rows = min(step, n - r0)
+        blk = np.empty((rows, n), dtype=np.float64, order="F")
+        fla.check(fla.lib().fla_hip_memcpy2d(blk.ctypes.data, rows * 8,
+                                              A0.ptr.value + r0 * 8, n * 8, rows * 8, n, 1))
+        best = max(best, float(np.max(np.sum(np.abs(blk), axis=1))))
+    return best
+
+
+def bench_batched(fla, np, count, seed=2026):
+    """4096 SPD matrices, n_i ~ U{32..512}, A_i = B_i B_i^T/n_i + 0.1 I (FP64 and
+    FP32), and 4096 U[-1,1] matrices for LU; each with 1-8 N(0,1) RHS; one
+    persistent launch each (autotuned configuration, graph replay)."""
+    import ctypes as C
+    lib = fla.lib()
+    rng = np.random.default_rng(seed)
+    n = rng.integers(32, 513, size=count)
+    r = rng.integers(1, 9, size=count)
+    flops_chol = float(np.sum(n.astype(np.float64) ** 3) / 3 + np.sum(2.0 * n * n * r))
+    flops_lu = float(np.sum(2.0 * n.astype(np.float64) ** 3 / 3) + np.sum(2.0 * n * n * r))
+    tot = int(np.sum(n * n))
+    totb = int(np.sum(n * r))
+    offs = np.concatenate([[0], np.cumsum(n * n)[:-1]])
+    offb = np.concatenate([[0], np.cumsum(n * r)[:-1]])
+    out = {"count": count, "n_range": [32, 512], "sum_n3_over_3": float(np.sum(n ** 3.0) / 3)}
+    A0 = fla.DeviceBuffer(tot * 8)
+    A = fla.DeviceBuffer(tot * 8)
+    B0 = fla.DeviceBuffer(totb * 8)
+    Bw = fla.DeviceBuffer(totb * 8)
+    info = fla.DeviceBuffer(count * 4)
+    piv = fla.DeviceBuffer(int(np.sum(n)) * 4)
+    pofs = np.concatenate([[0], np.cumsum(n)[:-1]])
+    Bg = fla.DeviceBuffer(512 * 512 * 8)
+    log(f"batched: generating {count} SPD matrices")
+    for i in range(count):
+        ni = int(n[i])
+        fla.check(lib.fla_hip_gen_normal(seed + i, ni, ni, Bg.ptr, ni))
+        fla.check(lib.fla_hip_gram_lower(ni, ni, Bg.ptr, ni, 1.0 / ni, 0.1,
+                                         C.c_void_p(A0.addr + int(offs[i]) * 8), ni))
+    fla.check(lib.fla_hip_gen_normal(seed + 99991, totb, 1, B0.ptr, totb))
+    fla.synchronize()
+    ni = (C.c_int * count)(*[int(v) for v in n])
+    ri = (C.c_int * count)(*[int(v) for v in r])
+
+    def ptrs(base, offsets, esz):
+        return (C.c_void_p * count)(*[base + int(o) * esz for o in offsets])
+
+    def run(fn, Aptrs, Bptrs, esz, steps=3, lu=False):
+        ts = []
+        for rep in range(steps + 1):
+            lib.fla_hip_memcpy_d2d(A.ptr, A0.ptr, tot * esz)
+            lib.fla_hip_memcpy_d2d(Bw.ptr, B0.ptr, totb * esz)
+            fla.synchronize()
+            t0 = time.perf_counter()
+            if lu:
+                fla.check(fn(count, ni, Aptrs, ni, ptrs(piv.addr, pofs, 4), Bptrs, ni, ri,
+                             info.ptr))
+            else:
+                fla.check(fn(count, ni, Aptrs, ni, Bptrs, ni, ri, info.ptr))
+            fla.synchronize()
+            if rep > 0:
+                ts.append(time.perf_counter() - t0)
+        inf = np.empty(count, dtype=np.int32)
+        lib.fla_hip_memcpy_d2h(inf.ctypes.data, info.ptr, count * 4)
+        return min(ts), int(np.sum(inf != -1))
+
+    for kind, name in ((0, "dposv"), (1, "sposv"), (2, "dgesv")):
+        esz = 4 if kind == 1 else 8
+        if kind == 1:  # FP32 copies of the same matrices and right-hand sides
+            A0d, B0d = A0, B0
+            A0, B0 = fla.DeviceBuffer(tot * 4), fla.DeviceBuffer(totb * 4)
+            fla.check(lib.fla_hip_convert_d2s(tot, 1, A0d.ptr, tot, A0.ptr, tot))
+            fla.check(lib.fla_hip_convert_d2s(totb, 1, B0d.ptr, totb, B0.ptr, totb))
+        if kind == 2:
+            A0, B0 = A0d, B0d
+            log("batched: generating U[-1,1] matrices for LU")
+            fla.check(lib.fla_hip_gen_uniform(seed + 7, tot, 1, A0.ptr, tot))
+            fla.check(lib.fla_hip_gen_normal(seed + 99991, totb, 1, B0.ptr, totb))
+        nb, wg, tms = C.c_int(), C.c_int(), C.c_double()
+        fla.check(lib.fla_hip_batched_tune(kind, count, ni, ri, C.byref(nb), C.byref(wg),
+                                           C.byref(tms)))
+        fn = {0: lib.fla_hip_dposv_batched, 1: lib.fla_hip_sposv_batched,
+              2: lib.fla_hip_dgesv_batched}[kind]
+        t, bad = run(fn, ptrs(A.addr, offs, esz), ptrs(Bw.addr, offb, esz), esz, lu=(kind == 2))
+        fl = flops_lu if kind == 2 else flops_chol
+        out[name] = {"ms": round(t * 1e3, 3), "tflops": round(fl / t / 1e12, 3),
+                     "failed": bad, "tuned_nb": nb.value, "tuned_wg_per_cu": wg.value}
+        log(f"batched {name}: {out[name]}")
+    del A0, A, B0, Bw, info, piv, Bg
+    return out
+
+
+def bench_adjoint(fla, np, n):
+    """GP log-marginal-likelihood gradient (RBF, l=0.2, s=1, noise 1e-3) at n
+    points in [0,1]^2: kernel matrix, FLA_Chol, solve, Cholesky adjoint,
+    solve adjoint, contraction.  Declared flops: n^3/3 (factor) + n^3/3
+    (L^T Lbar) + 2 n^3 (two n x n triangular solves)."""
+    import ctypes as C
+    lib = fla.lib()
+    rng = np.random.default_rng(8192)
+    x = np.asfortranarray(rng.uniform(0.0, 1.0, size=(n, 2)))
+    y = rng.standard_normal(n)
+    dx = fla.DeviceMatrix.from_host(x)
+    dy = fla.DeviceMatrix.from_host(y.reshape(-1, 1))
+    lml, g, info = C.c_double(), (C.c_double * 2)(), C.c_int()
+    ts = []
+    for rep in range(3):
+        fla.synchronize()
+        t0 = time.perf_counter()
+        fla.check(lib.fla_hip_gp_rbf_lml_grad(n, 2, dx.ptr, dx.ld, dy.ptr, 0.2, 1.0, 1e-3,
+                                              C.byref(lml), g, C.byref(info)))
+        ts.append(time.perf_counter() - t0)
+    t = min(ts[1:])
+    flops = n ** 3 / 3 + n ** 3 / 3 + 2.0 * n ** 3
+    out = {"n": n, "ms": round(t * 1e3, 2), "tflops_declared": round(flops / t / 1e12, 3),
+           "lml": lml.value, "grad_l": g[0], "grad_s": g[1], "info": info.value}
+    log(f"adjoint: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +304,12 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=3072)
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--mp-n", type=int, default=16384)
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--adj-n", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -194,6 +405,13 @@ def main():
                        "lu_update_kernel_tflops": round(gf / (gms * 1e-3) / 1e12, 3) if gms else None,
                        "lu_panel_ms_per_factorization": round(pms / len(lt), 2)})
         del L0, LA, piv
+
+    if not args.no_mixed:
+        extras["mixed"] = bench_mixed(fla, np, args.mp_n)
+    if not args.no_batched:
+        extras["batched"] = bench_batched(fla, np, args.batch_count)
+    if not args.no_adjoint:
+        extras["adjoint"] = bench_adjoint(fla, np, args.adj_n)
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     out = {

@@ -23,6 +23,7 @@
  *   FLA_Trsm/Herk/Syrk/Gemm_external  src/base/flamec/wrappers/blas/3/external/
  *   dpotrf_ / dgetrf_          src/map/lapack2flamec/FLA_potrf.c:31-105, FLA_getrf.c:32-117
  *   dpotrs_ / dgetrs_          src/map/common/lapacksrc/f2c/dpotrs.c:102, dgetrs.c:115
+ *   dsposv_                    src/map/common/lapacksrc/f2c/dsposv.c:192-415
  */
 #ifndef LIBFLAME_AMD_FLAME_H
 #define LIBFLAME_AMD_FLAME_H
@@ -215,6 +216,12 @@ FLA_Error FLA_LU_piv(FLA_Obj A, FLA_Obj p);
 FLA_Error FLA_Apply_pivots(FLA_Side side, FLA_Trans trans, FLA_Obj p, FLA_Obj A);
 FLA_Error FLA_Chol_solve(FLA_Uplo uplo, FLA_Obj A, FLA_Obj B, FLA_Obj X);
 FLA_Error FLA_LU_piv_solve(FLA_Obj A, FLA_Obj p, FLA_Obj B, FLA_Obj X);
+/* Reverse-mode adjoints (no counterpart in the reference; lower factor only):
+   Abar := (G + G')/2, G = L^-T Phi(L' Lbar) L^-1 (Phi: lower part, diagonal
+   halved); and for X = A^-1 B: Bbar := A^-1 Xbar, Abar -= (Bbar X' + X Bbar')/2. */
+FLA_Error FLA_Chol_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj Lbar, FLA_Obj Abar);
+FLA_Error FLA_Chol_solve_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj X, FLA_Obj Xbar, FLA_Obj Bbar,
+                                 FLA_Obj Abar);
 FLA_Error FLA_Shift_pivots_to(FLA_Pivot_type ptype, FLA_Obj p);
 FLA_Error FLA_Trsm_external(FLA_Side side, FLA_Uplo uplo, FLA_Trans trans, FLA_Diag diag,
                             FLA_Obj alpha, FLA_Obj A, FLA_Obj B);
@@ -236,6 +243,8 @@ int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* 
             int* info);
 int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b,
             int* ldb, int* info);
+int dsposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, double* x,
+            int* ldx, double* work, float* swork, int* iter, int* info);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,7 @@ extern "C" {
 #define FLA_HIP_EARG 9001   /* invalid argument */
 #define FLA_HIP_ENODEV 9002 /* no gfx950 device available */
 #define FLA_HIP_ESPIN 9003  /* a bounded inter-workgroup wait expired */
+#define FLA_HIP_ERANGE 9004 /* input does not fit the reduced precision */
 
 /* sides / uplo / trans / diag use the FLAME constants (FLAME.h). */
 
@@ -96,10 +97,100 @@ int fla_hip_set_gemm_config(int cfg);
    launch with HIP events on its stream and accumulate, per kind, the
    algorithmic flops and the elapsed milliseconds.
    kind 0 = Cholesky trailing update (lower SYRK), 1 = LU trailing update
-   (GEMM), 2 = LU panel, 3 = Cholesky panel chain. */
+   (GEMM), 2 = LU panel, 3 = Cholesky panel chain, 4 = reduced-precision
+   Cholesky trailing update (FP32 / BF16 MFMA). */
 int fla_hip_stats_enable(int on);
 int fla_hip_stats_reset(void);
 int fla_hip_stats_get(int kind, double* flops, double* ms, long* launches);
+
+/* ---- mixed precision (dsposv_ semantics, src/map/common/lapacksrc/f2c/dsposv.c:192) ---- */
+#define FLA_MP_FP32 0     /* FP32 trailing matrix, FP32 MFMA updates */
+#define FLA_MP_BF16 1     /* FP32 trailing matrix, BF16 MFMA updates, FP32 accumulation */
+#define FLA_IR_CLASSIC 0  /* x += (L L')^-1 r, as dsposv_ */
+#define FLA_IR_GMRES 1    /* GMRES-IR: GMRES on A d = r, preconditioned by (L L')^-1 */
+typedef struct {
+  int precision;     /* FLA_MP_FP32 (default) or FLA_MP_BF16 */
+  int refinement;    /* FLA_IR_CLASSIC (default) or FLA_IR_GMRES */
+  int max_iter;      /* refinement steps before the FP64 fall-back (default 30) */
+  int gmres_restart; /* Krylov dimension per GMRES-IR step (default 20) */
+  double gmres_tol;  /* relative preconditioned residual per GMRES-IR step (default 1e-4) */
+} fla_hip_mp_opts;
+/* Reduced-precision Cholesky of the symmetric A (uplo triangle): trailing
+   updates in FP32 or BF16 on MFMA, each panel factored in FP64 from the
+   reduced-precision trailing values.  Writes the lower factor (FP64 values)
+   into L; L's strictly upper part is not referenced.  *info = FLA_SUCCESS or
+   the 0-based failing column.  FLA_HIP_ERANGE if A overflows FP32. */
+int fla_hip_mp_chol(int uplo, int n, const double* A, int lda, int precision, double* L,
+                    int ldl, int* info);
+/* Solves A X = B for SPD A (uplo triangle) by the reduced-precision factor
+   and refinement with an FP64 residual, with dsposv_'s stopping rule,
+   iteration limit and FP64 fall-back.  *iter: number of refinement steps,
+   or -2 (overflow), -3 (factorization failed), -31 (no convergence), in
+   which case A holds the FP64 factor.  *info: 0, or k > 0 when the FP64
+   fall-back found the leading minor of order k not positive definite.
+   opts may be NULL (FP32, classic refinement, 30 steps). */
+int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* B, int ldb,
+                   double* X, int ldx, const fla_hip_mp_opts* opts, int* iter, int* info);
+/* Timings and counters of the last fla_hip_dsposv: [0] factor ms (incl.
+   conversion and norm), [1] refinement ms, [2] preconditioner solves,
+   [3] FP64 products with A, [4] ||A||_inf, [5] GMRES iterations, [6] the
+   reduced-precision factorization's info. */
+int fla_hip_mp_last_stats(double* out, int nout);
+
+/* ---- batched small factorizations and solves (no counterpart in the
+   reference: thousands of independent matrices of uneven sizes in one
+   persistent launch with a device work queue) ----
+   Host arrays n[], lda[], ldb[], nrhs[] and host arrays of DEVICE pointers
+   A[], B[], ipiv[]; info is a DEVICE int array[count] (FLA_SUCCESS or the
+   0-based failing column / first null pivot).  n[i] <= 512, nrhs[i] <= 256.
+   B may be NULL (factor only); with B, B[i] is overwritten by the solution
+   of A_i X = B_i.  Cholesky: lower triangle, FLA_Chol semantics per matrix;
+   LU: partial pivoting with FLAME-native pivots (FLA_LU_piv semantics).
+   Asynchronous on the library's stream; repeated calls with the same count
+   and configuration replay a cached hipGraph. */
+#define FLA_HIP_BATCH_MAXN 512
+int fla_hip_dposv_batched(int count, const int* n, double* const* A, const int* lda,
+                          double* const* B, const int* ldb, const int* nrhs, int* info);
+int fla_hip_sposv_batched(int count, const int* n, float* const* A, const int* lda,
+                          float* const* B, const int* ldb, const int* nrhs, int* info);
+int fla_hip_dgesv_batched(int count, const int* n, double* const* A, const int* lda,
+                          int* const* ipiv, double* const* B, const int* ldb, const int* nrhs,
+                          int* info);
+/* Autotuner: kind 0 = dposv, 1 = sposv, 2 = dgesv.  Times the candidate
+   configurations (panel width, workgroups per CU) on synthetic matrices of
+   these sizes and keeps the fastest for batches with the same size
+   histogram.  Outputs may be NULL. */
+int fla_hip_batched_tune(int kind, int count, const int* n, const int* nrhs, int* best_nb,
+                         int* best_wg_per_cu, double* best_ms);
+/* Forces one configuration for every batch of that kind (nb: 16 or 32 for
+   kinds 0 and 2, 32 or 64 for kind 1; 1 <= wg_per_cu <= 4). */
+int fla_hip_batched_set_config(int kind, int nb, int wg_per_cu);
+int fla_hip_batched_set_graph(int on);
+
+/* ---- adjoints (no counterpart in the reference) ----
+   Reverse mode of A = L L' (L lower, n x n): given Lbar (lower), Abar :=
+   (G + G')/2 with G = L^-T Phi(L' Lbar) L^-1, Phi = lower triangle with the
+   diagonal halved, so df = <Abar, dA> for symmetric dA.  The strictly upper
+   parts of L and Lbar are not referenced; Abar is written in full. */
+int fla_hip_dchol_adjoint(int n, const double* L, int ldl, const double* Lbar, int ldlb,
+                          double* Abar, int ldab);
+/* Reverse mode of X = A^-1 B with A = L L': Bbar := A^-1 Xbar and
+   Abar -= (Bbar X' + X Bbar')/2 (Abar accumulated). */
+int fla_hip_dchol_solve_adjoint(int n, int nrhs, const double* L, int ldl, const double* X,
+                                int ldx, const double* Xbar, int ldxb, double* Bbar, int ldbb,
+                                double* Abar, int ldab);
+/* Gaussian process with K = s^2 exp(-|x-x'|^2 / (2 l^2)) + noise*I on n
+   points x (n x d, column-major): the kernel matrix; the contraction
+   g = (<Kbar, dK/dl>, <Kbar, dK/ds>) into host g[2]; and the log marginal
+   likelihood of y (device vector) with its gradient in (l, s) (host lml,
+   grad[2]) through the Cholesky and Cholesky-solve adjoints.  info is the
+   Cholesky's (FLA_SUCCESS or the failing column). */
+int fla_hip_gp_rbf_kernel(int n, int d, const double* x, int ldx, double l, double s,
+                          double noise, double* K, int ldk);
+int fla_hip_gp_rbf_grad(int n, int d, const double* x, int ldx, double l, double s,
+                        const double* Kbar, int ldkb, double* g);
+int fla_hip_gp_rbf_lml_grad(int n, int d, const double* x, int ldx, const double* y, double l,
+                            double s, double noise, double* lml, double* grad, int* info);
 
 /* ---- synthetic inputs and checks (device side) ---- */
 /* Counter-based generators: identical values to the CPU generators. */
@@ -108,6 +199,12 @@ int fla_hip_gen_normal(unsigned long long seed, int m, int n, double* A, int lda
 /* A := scale*B*B' + shift*I (lower triangle and diagonal written). */
 int fla_hip_gram_lower(int n, int k, const double* B, int ldb, double scale,
                        double shift, double* A, int lda);
+/* SPD A = Q diag(s) Q' with s log-spaced on [1, kappa] and Q a product of
+   nrefl Householder reflectors with N(0,1) vectors (full matrix written). */
+int fla_hip_gen_spd_reflectors(unsigned long long seed, int n, int nrefl, double kappa,
+                               double* A, int lda);
+/* S := float(A), m x n (no overlap). */
+int fla_hip_convert_d2s(int m, int n, const double* A, int lda, float* S, int lds);
 /* A(i,:) *= 10^(-i/max(m,n)) */
 int fla_hip_scale_rows_logspace(int m, int n, double* A, int lda);
 /* Y := op(M)*X + beta*Y with M given by its uplo triangle as a symmetric

@@ -83,6 +83,8 @@ def lib():
             "dgetrf_": (C.c_int, [ip, ip, dp, ip, ip, ip]),
             "dpotrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, ip]),
             "dgetrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, ip, dp, ip, ip]),
+            "dsposv_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, dp, ip, dp,
+                                  C.POINTER(C.c_float), ip, ip]),
             "fla_hip_init": (C.c_int, []), "fla_hip_finalize": (C.c_int, []),
             "fla_hip_synchronize": (C.c_int, []),
             "fla_hip_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
@@ -114,6 +116,24 @@ def lib():
             "fla_hip_dmatmul_check": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                                 C.c_int, vp, C.c_int, vp, C.c_int, C.c_double,
                                                 vp, C.c_int]),
+            "fla_hip_gen_spd_reflectors": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int,
+                                                     C.c_double, vp, C.c_int]),
+            "fla_hip_mp_chol": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_int, vp, C.c_int,
+                                          ip]),
+            "fla_hip_dsposv": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int,
+                                         vp, C.c_int, C.POINTER(MpOpts), ip, ip]),
+            "fla_hip_mp_last_stats": (C.c_int, [dp, C.c_int]),
+            "fla_hip_dchol_adjoint": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int]),
+            "fla_hip_dchol_solve_adjoint": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, C.c_int,
+                                                      vp, C.c_int, vp, C.c_int, vp, C.c_int]),
+            "fla_hip_gp_rbf_kernel": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_double,
+                                                C.c_double, C.c_double, vp, C.c_int]),
+            "fla_hip_gp_rbf_grad": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_double,
+                                              C.c_double, vp, C.c_int, dp]),
+            "fla_hip_gp_rbf_lml_grad": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, C.c_double,
+                                                  C.c_double, C.c_double, dp, dp, ip]),
+            "FLA_Chol_adjoint": (C.c_int, [C.c_int, O, O, O]),
+            "FLA_Chol_solve_adjoint": (C.c_int, [C.c_int, O, O, O, O, O]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -121,6 +141,31 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+class MpOpts(C.Structure):
+    """fla_hip_mp_opts (FLA_hip.h)."""
+    _fields_ = [("precision", C.c_int), ("refinement", C.c_int), ("max_iter", C.c_int),
+                ("gmres_restart", C.c_int), ("gmres_tol", C.c_double)]
+
+
+FLA_MP_FP32, FLA_MP_BF16 = 0, 1
+FLA_IR_CLASSIC, FLA_IR_GMRES = 0, 1
+FLA_HIP_ERANGE = 9004
+
+
+def dsposv(uplo, A, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_iter=30,
+           gmres_restart=20, gmres_tol=1e-4):
+    """fla_hip_dsposv on device matrices; returns (iter, info, stats dict)."""
+    o = MpOpts(precision, refinement, max_iter, gmres_restart, gmres_tol)
+    it, info = C.c_int(), C.c_int()
+    check(lib().fla_hip_dsposv(uplo, A.m, B.n, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld,
+                               C.byref(o), C.byref(it), C.byref(info)), "fla_hip_dsposv")
+    st = (C.c_double * 8)()
+    check(lib().fla_hip_mp_last_stats(st, 8))
+    keys = ["factor_ms", "refine_ms", "prec_solves", "matvecs", "anorm", "gmres_iters",
+            "lp_info"]
+    return it.value, info.value, {k: st[i] for i, k in enumerate(keys)}
 
 
 def check(status, what="libflame_amd call"):
@@ -240,6 +285,14 @@ def FLA_Chol_solve(uplo, A, B, X):
     return lib().FLA_Chol_solve(uplo, A.o, B.o, X.o)
 
 
+def FLA_Chol_adjoint(uplo, L, Lbar, Abar):
+    return lib().FLA_Chol_adjoint(uplo, L.o, Lbar.o, Abar.o)
+
+
+def FLA_Chol_solve_adjoint(uplo, L, X, Xbar, Bbar, Abar):
+    return lib().FLA_Chol_solve_adjoint(uplo, L.o, X.o, Xbar.o, Bbar.o, Abar.o)
+
+
 def FLA_LU_piv_solve(A, p, B, X):
     return lib().FLA_LU_piv_solve(A.o, p.o, B.o, X.o)
 
@@ -257,6 +310,7 @@ def set_blocksizes(chol_nb=0, lu_nb=0):
 
 
 STATS_CHOL_UPDATE, STATS_LU_UPDATE, STATS_LU_PANEL, STATS_CHOL_PANEL = 0, 1, 2, 3
+STATS_MP_UPDATE = 4
 
 
 def stats_enable(on=True):

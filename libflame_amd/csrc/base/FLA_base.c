@@ -187,6 +187,12 @@ FLA_Error fla_check_int_object(FLA_Obj A) {
 FLA_Error fla_check_square(FLA_Obj A) {
   return FLA_Obj_length(A) == FLA_Obj_width(A) ? FLA_SUCCESS : FLA_OBJECT_NOT_SQUARE;
 }
+FLA_Error fla_check_conformal_dims(FLA_Trans trans, FLA_Obj A, FLA_Obj B) {
+  const dim_t m = (trans == FLA_NO_TRANSPOSE) ? FLA_Obj_length(A) : FLA_Obj_width(A);
+  const dim_t n = (trans == FLA_NO_TRANSPOSE) ? FLA_Obj_width(A) : FLA_Obj_length(A);
+  return (m == FLA_Obj_length(B) && n == FLA_Obj_width(B)) ? FLA_SUCCESS
+                                                           : FLA_NONCONFORMAL_DIMENSIONS;
+}
 FLA_Error fla_check_col_vector(FLA_Obj A) {
   return FLA_Obj_width(A) == 1 ? FLA_SUCCESS : FLA_OBJECT_NOT_VECTOR;
 }

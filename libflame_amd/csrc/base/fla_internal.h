@@ -15,6 +15,7 @@ FLA_Error fla_check_floating_object(FLA_Obj A);
 FLA_Error fla_check_nonconstant_object(FLA_Obj A);
 FLA_Error fla_check_int_object(FLA_Obj A);
 FLA_Error fla_check_square(FLA_Obj A);
+FLA_Error fla_check_conformal_dims(FLA_Trans trans, FLA_Obj A, FLA_Obj B);
 FLA_Error fla_check_col_vector(FLA_Obj A);
 FLA_Error fla_check_if_vector(FLA_Obj A);
 FLA_Error fla_check_vector_dim_min(FLA_Obj x, dim_t dim);

@@ -1,0 +1,819 @@
+// Batched factorizations and solves of many small independent matrices of
+// uneven sizes (n <= 512) in one persistent launch.
+//
+// Every workgroup pulls matrix indices from a device work counter (the batch
+// is ordered by decreasing n, so the large matrices start first and the
+// small ones fill the tail), factors its matrix alone and, when right-hand
+// sides are given, solves with them before taking the next index.  The
+// algorithms follow the right-looking blocked forms of FLA_Chol_l_blk_var3
+// (src/lapack/dec/chol/l/flamec/FLA_Chol_l_blk_var3.c) and FLA_LU_piv_blk_var5
+// (src/lapack/dec/lu/piv/vars/flamec/FLA_LU_piv_blk_var5.c) with a panel of
+// NB columns held in LDS:
+//   Cholesky: unblocked right-looking factor of the m x NB panel in LDS
+//     (dpotf2 order: sqrt, invscal, rank-1 update), then the trailing update
+//     A22 -= L21 L21^T on MFMA (lower 32 x 32 tiles, read-modify-write).
+//   LU: per column of the panel the pivot is the first maximal |a| of the
+//     column (idamax_, src/flablas/f2c/idamax.c:40-46, NaN rule included),
+//     rows are swapped across the whole matrix (FLAME-native relative
+//     pivots), the column scaled by the reciprocal of the pivot and the
+//     panel updated; then A12 := inv(L11) A12 and A22 -= L21 A12 on MFMA.
+// The forward substitution of the solve runs inside the factorization,
+// panel by panel, while L's panel is in LDS; the backward substitution reads
+// the factor once more from memory.
+//
+// Element types: double (v_mfma_f64_16x16x4f64) and float
+// (v_mfma_f32_16x16x4f32).  Info per matrix: FLA_SUCCESS, or the 0-based
+// failing column (Cholesky) / first null pivot (LU, factorization continues
+// as FLA_LU_piv does).
+#include "fla_hip_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace {
+
+constexpr int MAXN = 512;
+
+struct BatchItem {
+  void* A;
+  void* B;
+  int* ipiv;
+  long lda, ldb;
+  int n, nrhs, index;
+  int pad;
+};
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// MFMA 16x16x4 with the column fragment as the A operand and the row
+// fragment as B: lane (li = l%16, lk = l/16), register r holds
+// C(row0 + li, col0 + dj(lk, r)).
+__device__ __forceinline__ d4 mfma4(double colf, double rowf, d4 acc) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(colf, rowf, acc, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma4(float colf, float rowf, f4 acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(colf, rowf, acc, 0, 0, 0);
+}
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+  typedef d4 type;
+  static __device__ __forceinline__ int dj(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct Acc<float> {
+  typedef f4 type;
+  static __device__ __forceinline__ int dj(int lk, int r) { return 4 * lk + r; }
+};
+
+__device__ __forceinline__ double fabs_t(double x) { return fabs(x); }
+__device__ __forceinline__ float fabs_t(float x) { return fabsf(x); }
+
+// idamax_ summary of a contiguous run of a column: the last NaN position and
+// the first maximal |x| after it.
+struct AmaxSum {
+  double best;
+  int bidx;  // -1: no element after the last NaN
+  int nidx;  // -1: no NaN
+};
+__device__ __forceinline__ AmaxSum amax_combine(const AmaxSum& L, const AmaxSum& R) {
+  if (R.nidx >= 0) return R;
+  AmaxSum o = L;
+  if (R.bidx >= 0 && (o.bidx < 0 || R.best > o.best)) {
+    o.best = R.best;
+    o.bidx = R.bidx;
+  }
+  return o;
+}
+
+// Workgroup-wide idamax over rows [r0, m) of LDS column col (ld MAXN).
+// 256 threads, each a contiguous run of rows; result row (absolute) in *out.
+template <typename T>
+__device__ int block_idamax(const T* col, int r0, int m, AmaxSum* red, int t) {
+  const int len = m - r0;
+  const int per = (len + 255) / 256;
+  AmaxSum s{0.0, -1, -1};
+  const int b = r0 + t * per, e = min(m, b + per);
+  for (int i = b; i < e; ++i) {
+    const double a = (double)fabs_t(col[i]);
+    if (a != a) {
+      s.nidx = i;
+      s.bidx = -1;
+    } else if (s.bidx < 0 || a > s.best) {
+      s.best = a;
+      s.bidx = i;
+    }
+  }
+  red[t] = s;
+  __syncthreads();
+  for (int st = 1; st < 256; st <<= 1) {
+    AmaxSum v;
+    const bool act = (t % (2 * st)) == 0 && t + st < 256;
+    if (act) v = amax_combine(red[t], red[t + st]);
+    __syncthreads();
+    if (act) red[t] = v;
+    __syncthreads();
+  }
+  const AmaxSum r = red[0];
+  __syncthreads();
+  return r.bidx >= 0 ? r.bidx : r.nidx;
+}
+
+// Trailing update C := C - Lr * Lc^T on MFMA, 32 x 32 tiles (lower tiles
+// only when LOWER) of the m2 x n2 block C (ld ldc).  Row operand: LDS panel
+// rows (element (row, kk) at Pr[row + kk*MAXN]); column operand element
+// (col, kk) at Pc[col*pcs + kk*pks] (the LDS panel for the Cholesky, U12 in
+// memory for the LU).
+template <typename T, bool LOWER>
+__device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, long pcs, long pks,
+                            T* C, long ldc, int t) {
+  typedef typename Acc<T>::type AT;
+  const int w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+  const int T1 = (m2 + 31) / 32, T2 = (n2 + 31) / 32;
+  const int ntiles = LOWER ? T1 * (T1 + 1) / 2 : T1 * T2;
+  for (int tile = w; tile < ntiles; tile += 4) {
+    int ti, tj;
+    if (LOWER) {
+      tj = 0;
+      int rem = tile;
+      while (rem >= T1 - tj) {
+        rem -= T1 - tj;
+        ++tj;
+      }
+      ti = tj + rem;
+    } else {
+      ti = tile % T1;
+      tj = tile / T1;
+    }
+    AT acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a][b][r] = (T)0;
+    for (int ks = 0; ks < kb; ks += 4) {
+      const int kk = ks + lk;
+      T rf[2], cf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int row = ti * 32 + a * 16 + li;
+        rf[a] = (row < m2 && kk < kb) ? Pr[row + kk * MAXN] : (T)0;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int col = tj * 32 + b * 16 + li;
+        cf[b] = (col < n2 && kk < kb) ? Pc[(long)col * pcs + (long)kk * pks] : (T)0;
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma4(cf[b], rf[a], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 32 + a * 16 + li;
+          const int j = tj * 32 + b * 16 + Acc<T>::dj(lk, r);
+          if (i < m2 && j < n2 && (!LOWER || i >= j)) C[i + (long)j * ldc] -= acc[a][b][r];
+        }
+  }
+}
+
+// ---- Cholesky of one matrix (lower), fused forward substitution -----------
+template <typename T, int NB>
+__device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, int t) {
+  int fail = -1;
+  for (int k = 0; k < n; k += NB) {
+    const int kb = min(NB, n - k), m = n - k;
+    // panel A[k:n, k:k+kb] -> P (ld MAXN)
+    for (int e = t; e < m * kb; e += 256) {
+      const int r = e % m, c = e / m;
+      P[r + c * MAXN] = A[(k + r) + (long)(k + c) * lda];
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+      const T d = P[j + j * MAXN];
+      if (!(d > (T)0)) {
+        fail = k + j;
+        break;
+      }
+      const T sd = sqrt(d);
+      const T inv = (T)1 / sd;
+      for (int r = j + 1 + t; r < m; r += 256) P[r + j * MAXN] *= inv;
+      __syncthreads();
+      if (t == 0) P[j + j * MAXN] = sd;
+      const int rr = m - j - 1, cc = kb - j - 1;
+      for (int e = t; e < rr * cc; e += 256) {
+        const int r = j + 1 + e % rr, c = j + 1 + e / rr;
+        if (r >= c) P[r + c * MAXN] -= P[r + j * MAXN] * P[c + j * MAXN];
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    // write the factored panel back (lower part)
+    for (int e = t; e < m * kb; e += 256) {
+      const int r = e % m, c = e / m;
+      if (r >= c) A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
+    }
+    if (fail >= 0) return fail;
+    // forward substitution with this panel: Y_k = L_kk^-1 B_k, B_rest -= L21 Y_k
+    if (nrhs > 0) {
+      for (int j = 0; j < kb; ++j) {
+        for (int c = t; c < nrhs; c += 256) B[(k + j) + (long)c * ldb] /= P[j + j * MAXN];
+        __syncthreads();
+        for (int e = t; e < (kb - j - 1) * nrhs; e += 256) {
+          const int r = j + 1 + e % (kb - j - 1), c = e / (kb - j - 1);
+          B[(k + r) + (long)c * ldb] -= P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
+        }
+        __syncthreads();
+      }
+      for (int e = t; e < (m - kb) * nrhs; e += 256) {
+        const int r = kb + e % (m - kb), c = e / (m - kb);
+        T s = (T)0;
+        for (int j = 0; j < kb; ++j) s += P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
+        B[(k + r) + (long)c * ldb] -= s;
+      }
+    }
+    // trailing update
+    if (m > kb)
+      tile_update<T, true>(m - kb, m - kb, kb, P + kb, P + kb, 1, MAXN,
+                           A + (k + kb) + (long)(k + kb) * lda, lda, t);
+    __syncthreads();
+  }
+  return fail;
+}
+
+// Backward substitution X := L^-T Y (L lower in A), block by block from the
+// bottom, NB rows at a time.
+template <typename T, int NB>
+__device__ void chol_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* S, int t) {
+  const int nbk = (n + NB - 1) / NB;
+  for (int q = nbk - 1; q >= 0; --q) {
+    const int k = q * NB, kb = min(NB, n - k);
+    // B_k -= L[k+kb:n, k:k+kb]^T X[k+kb:n]
+    for (int e = t; e < kb * nrhs; e += 256) {
+      const int j = e % kb, c = e / kb;
+      T s = (T)0;
+      for (int i = k + kb; i < n; ++i) s += A[i + (long)(k + j) * lda] * B[i + (long)c * ldb];
+      S[j + c * NB] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < kb * nrhs; e += 256) {
+      const int j = e % kb, c = e / kb;
+      B[(k + j) + (long)c * ldb] -= S[j + c * NB];
+    }
+    __syncthreads();
+    // X_k = L_kk^-T B_k (backward column sweep)
+    for (int j = kb - 1; j >= 0; --j) {
+      for (int c = t; c < nrhs; c += 256) B[(k + j) + (long)c * ldb] /= A[(k + j) + (long)(k + j) * lda];
+      __syncthreads();
+      for (int e = t; e < j * nrhs; e += 256) {
+        const int i = e % j, c = e / j;
+        B[(k + i) + (long)c * ldb] -= A[(k + j) + (long)(k + i) * lda] * B[(k + j) + (long)c * ldb];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---- LU with partial pivoting of one square matrix, fused forward solve ----
+template <typename T, int NB>
+__device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs, T* P,
+                      AmaxSum* red, int* spiv, int t) {
+  int first_null = -1;
+  for (int k = 0; k < n; k += NB) {
+    const int kb = min(NB, n - k), m = n - k;
+    for (int e = t; e < m * kb; e += 256) {
+      const int r = e % m, c = e / m;
+      P[r + c * MAXN] = A[(k + r) + (long)(k + c) * lda];
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+      const int pr = block_idamax<T>(P + j * MAXN, j, m, red, t);
+      if (t == 0) spiv[j] = pr - j;  // relative pivot
+      // swap rows j and pr inside the panel
+      if (pr != j)
+        for (int c = t; c < kb; c += 256) {
+          const T a = P[j + c * MAXN], b = P[pr + c * MAXN];
+          P[j + c * MAXN] = b;
+          P[pr + c * MAXN] = a;
+        }
+      __syncthreads();
+      const T piv = P[j + j * MAXN];
+      if (piv != (T)0) {
+        const T inv = (T)1 / piv;
+        for (int r = j + 1 + t; r < m; r += 256) P[r + j * MAXN] *= inv;
+      } else if (first_null < 0) {
+        first_null = k + j;
+      }
+      __syncthreads();
+      const int rr = m - j - 1, cc = kb - j - 1;
+      for (int e = t; e < rr * cc; e += 256) {
+        const int r = j + 1 + e % rr, c = j + 1 + e / rr;
+        P[r + c * MAXN] -= P[r + j * MAXN] * P[j + c * MAXN];
+      }
+      __syncthreads();
+    }
+    // panel back to A; pivots out; swaps applied to the columns left and right
+    for (int e = t; e < m * kb; e += 256) {
+      const int r = e % m, c = e / m;
+      A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
+    }
+    for (int j = t; j < kb; j += 256) ipiv[k + j] = spiv[j];
+    for (int c = t; c < n; c += 256) {
+      if (c >= k && c < k + kb) continue;
+      for (int j = 0; j < kb; ++j) {
+        const int pr = k + j + spiv[j];
+        if (pr != k + j) {
+          const T a = A[(k + j) + (long)c * lda];
+          A[(k + j) + (long)c * lda] = A[pr + (long)c * lda];
+          A[pr + (long)c * lda] = a;
+        }
+      }
+    }
+    if (nrhs > 0)
+      for (int c = t; c < nrhs; c += 256)
+        for (int j = 0; j < kb; ++j) {
+          const int pr = k + j + spiv[j];
+          if (pr != k + j) {
+            const T a = B[(k + j) + (long)c * ldb];
+            B[(k + j) + (long)c * ldb] = B[pr + (long)c * ldb];
+            B[pr + (long)c * ldb] = a;
+          }
+        }
+    __syncthreads();
+    const int n2 = n - k - kb;
+    if (n2 > 0) {
+      // A12 := inv(L11) A12 (unit lower), one thread per column
+      for (int c = t; c < n2; c += 256) {
+        T x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) x[i] = (i < kb) ? A[(k + i) + (long)(k + kb + c) * lda] : (T)0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          // x[j] is final here; keeps later columns' LDS reads from being
+          // hoisted (hundreds of live values would spill)
+          asm volatile("" : "+v"(x[j])::"memory");
+#pragma unroll
+          for (int i = j + 1; i < NB; ++i)
+            if (i < kb && j < kb) x[i] -= P[i + j * MAXN] * x[j];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+          if (i < kb) A[(k + i) + (long)(k + kb + c) * lda] = x[i];
+      }
+      __threadfence_block();
+      __syncthreads();
+      const T* U12 = A + k + (long)(k + kb) * lda;  // (kk, col) at U12[kk + col*lda]
+      tile_update<T, false>(m - kb, n2, kb, P + kb, U12, lda, 1,
+                            A + (k + kb) + (long)(k + kb) * lda, lda, t);
+    }
+    // forward substitution of the solve (unit lower), rows already permuted
+    if (nrhs > 0) {
+      for (int j = 0; j < kb; ++j) {
+        __syncthreads();
+        for (int e = t; e < (kb - j - 1) * nrhs; e += 256) {
+          const int r = j + 1 + e % (kb - j - 1), c = e / (kb - j - 1);
+          B[(k + r) + (long)c * ldb] -= P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
+        }
+      }
+      __syncthreads();
+      for (int e = t; e < (m - kb) * nrhs; e += 256) {
+        const int r = kb + e % (m - kb), c = e / (m - kb);
+        T s = (T)0;
+        for (int j = 0; j < kb; ++j) s += P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
+        B[(k + r) + (long)c * ldb] -= s;
+      }
+    }
+    __syncthreads();
+  }
+  return first_null;
+}
+
+// X := U^-1 Y (U upper in A), NB rows at a time from the bottom.
+template <typename T, int NB>
+__device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* S, int t) {
+  const int nbk = (n + NB - 1) / NB;
+  for (int q = nbk - 1; q >= 0; --q) {
+    const int k = q * NB, kb = min(NB, n - k);
+    for (int e = t; e < kb * nrhs; e += 256) {
+      const int j = e % kb, c = e / kb;
+      T s = (T)0;
+      for (int i = k + kb; i < n; ++i) s += A[(k + j) + (long)i * lda] * B[i + (long)c * ldb];
+      S[j + c * NB] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < kb * nrhs; e += 256) {
+      const int j = e % kb, c = e / kb;
+      B[(k + j) + (long)c * ldb] -= S[j + c * NB];
+    }
+    __syncthreads();
+    for (int j = kb - 1; j >= 0; --j) {
+      for (int c = t; c < nrhs; c += 256) B[(k + j) + (long)c * ldb] /= A[(k + j) + (long)(k + j) * lda];
+      __syncthreads();
+      for (int e = t; e < j * nrhs; e += 256) {
+        const int i = e % j, c = e / j;
+        B[(k + i) + (long)c * ldb] -= A[(k + i) + (long)(k + j) * lda] * B[(k + j) + (long)c * ldb];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Persistent driver: OP 0 Cholesky (lower), 1 LU with partial pivoting.
+template <typename T, int OP, int NB>
+__global__ __launch_bounds__(256) void batched_kernel(const BatchItem* __restrict__ items,
+                                                      int count, int* counter, int* info,
+                                                      int max_rhs) {
+  __shared__ T P[MAXN * NB];
+  __shared__ T S[NB * 8];
+  __shared__ AmaxSum red[OP == 1 ? 256 : 1];
+  __shared__ int spiv[NB];
+  __shared__ int cur;
+  const int t = threadIdx.x;
+  (void)max_rhs;
+  for (;;) {
+    if (t == 0) cur = atomicAdd(counter, 1);
+    __syncthreads();
+    const int idx = cur;
+    __syncthreads();
+    if (idx >= count) break;
+    const BatchItem it = items[idx];
+    T* A = static_cast<T*>(it.A);
+    T* B = static_cast<T*>(it.B);
+    const int nr = it.B ? it.nrhs : 0;
+    int r;
+    if (OP == 0) {
+      r = chol_one<T, NB>(A, it.lda, it.n, B, it.ldb, nr, P, t);
+      __syncthreads();
+      if (r < 0 && nr > 0) {
+        for (int c0 = 0; c0 < nr; c0 += 8)
+          chol_back<T, NB>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), S, t);
+      }
+    } else {
+      r = lu_one<T, NB>(A, it.lda, it.n, it.ipiv, B, it.ldb, nr, P, red, spiv, t);
+      __syncthreads();
+      if (r < 0 && nr > 0) {
+        for (int c0 = 0; c0 < nr; c0 += 8)
+          lu_back<T, NB>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), S, t);
+      }
+    }
+    if (t == 0) info[it.index] = r < 0 ? kSuccess : r;
+    __syncthreads();
+  }
+}
+
+__global__ void reset_counter_kernel(int* c) { *c = 0; }
+
+// ---- host side: schedule, configurations, autotuner, graph replay ----------
+
+struct BatchConfig {
+  int nb = 32;        // panel width
+  int wg_per_cu = 1;  // persistent workgroups per CU
+};
+
+enum { kDpotrf = 0, kSpotrf = 1, kDgetrf = 2, kKinds = 3 };
+
+// Size signature of a batch: counts per 64-wide size bucket (n <= 512) and
+// the presence of right-hand sides.
+typedef std::vector<int> Signature;
+
+struct TunedEntry {
+  Signature sig;
+  BatchConfig cfg;
+};
+std::vector<TunedEntry> g_tuned[kKinds];
+bool g_use_graph = true;
+
+struct GraphCache {
+  bool valid = false;
+  int kind = -1, count = 0;
+  BatchConfig cfg;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+GraphCache g_graph;
+
+// Device and pinned staging of the items (grow-only).
+BatchItem* g_items_dev = nullptr;
+BatchItem* g_items_pin = nullptr;
+int* g_counter = nullptr;
+size_t g_items_cap = 0;
+
+Signature signature_of(int count, const int* n, const int* nrhs) {
+  Signature s(10, 0);
+  for (int i = 0; i < count; ++i) s[std::min(8, n[i] / 64)]++;
+  bool rhs = false;
+  for (int i = 0; nrhs && i < count; ++i) rhs |= nrhs[i] > 0;
+  s[9] = rhs ? 1 : 0;
+  return s;
+}
+
+BatchConfig default_config(int kind) {
+  BatchConfig c;
+  c.nb = 32;
+  c.wg_per_cu = (kind == kSpotrf) ? 2 : 1;
+  return c;
+}
+
+// A tuned entry for this exact signature, else a forced configuration
+// (empty signature, fla_hip_batched_set_config), else the default.
+BatchConfig lookup_config(int kind, const Signature& sig) {
+  for (const TunedEntry& e : g_tuned[kind])
+    if (e.sig == sig) return e.cfg;
+  for (const TunedEntry& e : g_tuned[kind])
+    if (e.sig.empty()) return e.cfg;
+  return default_config(kind);
+}
+
+template <typename T, int OP, int NB>
+int launch_batched(hipStream_t s, int grid, const BatchItem* items, int count, int* counter,
+                   int* info) {
+  hipLaunchKernelGGL((batched_kernel<T, OP, NB>), dim3(grid), dim3(256), 0, s, items, count,
+                     counter, info, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+int launch_kind(hipStream_t s, int kind, const BatchConfig& cfg, int count, int* info) {
+  const int grid = std::max(1, std::min(count, fla_ctx().num_cus * cfg.wg_per_cu));
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, g_counter);
+  FLA_HIP_CHECK(hipGetLastError());
+  if (kind == kDpotrf) {
+    if (cfg.nb == 16) return launch_batched<double, 0, 16>(s, grid, g_items_dev, count, g_counter, info);
+    return launch_batched<double, 0, 32>(s, grid, g_items_dev, count, g_counter, info);
+  }
+  if (kind == kSpotrf) {
+    if (cfg.nb == 64) return launch_batched<float, 0, 64>(s, grid, g_items_dev, count, g_counter, info);
+    return launch_batched<float, 0, 32>(s, grid, g_items_dev, count, g_counter, info);
+  }
+  if (cfg.nb == 16) return launch_batched<double, 1, 16>(s, grid, g_items_dev, count, g_counter, info);
+  return launch_batched<double, 1, 32>(s, grid, g_items_dev, count, g_counter, info);
+}
+
+bool valid_nb(int kind, int nb) {
+  if (kind == kSpotrf) return nb == 32 || nb == 64;
+  return nb == 16 || nb == 32;
+}
+
+// Validates the arguments and stages the items, largest n first.
+int stage_items(int count, const int* n, void* const* A, const int* lda, int* const* ipiv,
+                void* const* B, const int* ldb, const int* nrhs, bool need_ipiv) {
+  if (count < 0 || (count > 0 && (!n || !A || !lda))) return FLA_HIP_EARG;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < 0 || n[i] > MAXN || lda[i] < std::max(1, n[i]) || (n[i] > 0 && !A[i]))
+      return FLA_HIP_EARG;
+    if (need_ipiv && n[i] > 0 && (!ipiv || !ipiv[i])) return FLA_HIP_EARG;
+    if (B && nrhs && nrhs[i] > 0) {
+      if (!B[i] || !ldb || ldb[i] < std::max(1, n[i]) || nrhs[i] > 256) return FLA_HIP_EARG;
+    }
+  }
+  if ((size_t)count > g_items_cap) {
+    FLA_HIP_CHECK(hipDeviceSynchronize());
+    if (g_items_dev) FLA_HIP_CHECK(hipFree(g_items_dev));
+    if (g_items_pin) FLA_HIP_CHECK(hipHostFree(g_items_pin));
+    g_items_dev = nullptr;
+    g_items_pin = nullptr;
+    g_graph.valid = false;
+    const size_t cap = std::max<size_t>(1024, count);
+    FLA_HIP_CHECK(hipMalloc(&g_items_dev, cap * sizeof(BatchItem)));
+    FLA_HIP_CHECK(hipHostMalloc(&g_items_pin, cap * sizeof(BatchItem), hipHostMallocDefault));
+    g_items_cap = cap;
+  }
+  if (!g_counter) FLA_HIP_CHECK(hipMalloc(&g_counter, 256));
+  std::vector<int> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+  // the previous launch may still read the pinned items
+  FLA_HIP_CHECK(hipStreamSynchronize(fla_ctx().s_update));
+  for (int q = 0; q < count; ++q) {
+    const int i = order[q];
+    BatchItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.A = A[i];
+    it.lda = lda[i];
+    it.n = n[i];
+    it.index = i;
+    it.ipiv = need_ipiv ? ipiv[i] : nullptr;
+    const int r = (B && nrhs) ? nrhs[i] : 0;
+    it.nrhs = r;
+    it.B = r > 0 ? B[i] : nullptr;
+    it.ldb = r > 0 ? ldb[i] : 1;
+    g_items_pin[q] = it;
+  }
+  return FLA_HIP_OK;
+}
+
+// Copies the staged items and runs the persistent kernel, through a cached
+// hipGraph (memcpy node, counter reset kernel, kernel) when enabled.
+int run_staged(int kind, int count, const BatchConfig& cfg, int* info) {
+  hipStream_t s = fla_ctx().s_update;
+  if (count == 0) return FLA_HIP_OK;
+  if (!g_use_graph) {
+    FLA_HIP_CHECK(hipMemcpyAsync(g_items_dev, g_items_pin, sizeof(BatchItem) * count,
+                                 hipMemcpyHostToDevice, s));
+    return launch_kind(s, kind, cfg, count, info);
+  }
+  const bool hit = g_graph.valid && g_graph.kind == kind && g_graph.count == count &&
+                   g_graph.cfg.nb == cfg.nb && g_graph.cfg.wg_per_cu == cfg.wg_per_cu;
+  static int* g_graph_info = nullptr;
+  if (!hit || g_graph_info != info) {
+    if (g_graph.exec) FLA_HIP_CHECK(hipGraphExecDestroy(g_graph.exec));
+    if (g_graph.graph) FLA_HIP_CHECK(hipGraphDestroy(g_graph.graph));
+    g_graph.exec = nullptr;
+    g_graph.graph = nullptr;
+    g_graph.valid = false;
+    hipStream_t cs;
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    FLA_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int st = FLA_HIP_OK;
+    if (hipMemcpyAsync(g_items_dev, g_items_pin, sizeof(BatchItem) * count,
+                       hipMemcpyHostToDevice, cs) != hipSuccess)
+      st = FLA_HIP_EARG;
+    if (st == FLA_HIP_OK) st = launch_kind(cs, kind, cfg, count, info);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    FLA_HIP_CHECK(hipStreamDestroy(cs));
+    if (st != FLA_HIP_OK) return st;
+    FLA_HIP_CHECK(ec);
+    g_graph.graph = g;
+    FLA_HIP_CHECK(hipGraphInstantiate(&g_graph.exec, g, nullptr, nullptr, 0));
+    g_graph.valid = true;
+    g_graph.kind = kind;
+    g_graph.count = count;
+    g_graph.cfg = cfg;
+    g_graph_info = info;
+  }
+  FLA_HIP_CHECK(hipGraphLaunch(g_graph.exec, s));
+  return FLA_HIP_OK;
+}
+
+int batched_entry(int kind, int count, const int* n, void* const* A, const int* lda,
+                  int* const* ipiv, void* const* B, const int* ldb, const int* nrhs, int* info) {
+  FLA_TRY(fla_ensure_ready());
+  if (count > 0 && !info) return FLA_HIP_EARG;
+  FLA_TRY(stage_items(count, n, A, lda, ipiv, B, ldb, nrhs, kind == kDgetrf));
+  const BatchConfig cfg = lookup_config(kind, signature_of(count, n, B ? nrhs : nullptr));
+  return run_staged(kind, count, cfg, info);
+}
+
+// ---- autotuner ----------------------------------------------------------------
+
+__global__ void tune_fill_kernel(int n, double* A, float* Af, int spd, unsigned long long seed) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  unsigned long long z = seed + (unsigned long long)(spd ? (i > j ? i * 1315423911ULL + j : j * 1315423911ULL + i) : idx);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  double v = (double)(z >> 11) * 0x1.0p-53 - 0.5;
+  if (spd && i == j) v = (double)n;
+  if (A) A[idx] = v;
+  if (Af) Af[idx] = (float)v;
+}
+
+}  // namespace
+
+extern "C" int fla_hip_dposv_batched(int count, const int* n, double* const* A, const int* lda,
+                                     double* const* B, const int* ldb, const int* nrhs,
+                                     int* info) {
+  return batched_entry(kDpotrf, count, n, reinterpret_cast<void* const*>(A), lda, nullptr,
+                       reinterpret_cast<void* const*>(B), ldb, nrhs, info);
+}
+
+extern "C" int fla_hip_sposv_batched(int count, const int* n, float* const* A, const int* lda,
+                                     float* const* B, const int* ldb, const int* nrhs, int* info) {
+  return batched_entry(kSpotrf, count, n, reinterpret_cast<void* const*>(A), lda, nullptr,
+                       reinterpret_cast<void* const*>(B), ldb, nrhs, info);
+}
+
+extern "C" int fla_hip_dgesv_batched(int count, const int* n, double* const* A, const int* lda,
+                                     int* const* ipiv, double* const* B, const int* ldb,
+                                     const int* nrhs, int* info) {
+  return batched_entry(kDgetrf, count, n, reinterpret_cast<void* const*>(A), lda, ipiv,
+                       reinterpret_cast<void* const*>(B), ldb, nrhs, info);
+}
+
+extern "C" int fla_hip_batched_set_graph(int on) {
+  g_use_graph = on != 0;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_batched_set_config(int kind, int nb, int wg_per_cu) {
+  if (kind < 0 || kind >= kKinds || !valid_nb(kind, nb) || wg_per_cu < 1 || wg_per_cu > 4)
+    return FLA_HIP_EARG;
+  g_tuned[kind].clear();
+  BatchConfig c;
+  c.nb = nb;
+  c.wg_per_cu = wg_per_cu;
+  g_tuned[kind].push_back({Signature(), c});  // empty signature: see lookup below
+  return FLA_HIP_OK;
+}
+
+// Times every candidate configuration on synthetic matrices of the given
+// sizes (SPD for the Cholesky kinds, random for the LU) and keeps the
+// fastest for this batch's size signature.  Returns the choice.
+extern "C" int fla_hip_batched_tune(int kind, int count, const int* n, const int* nrhs,
+                                    int* best_nb, int* best_wg, double* best_ms) {
+  FLA_TRY(fla_ensure_ready());
+  if (kind < 0 || kind >= kKinds || count <= 0 || !n) return FLA_HIP_EARG;
+  hipStream_t s = fla_ctx().s_update;
+  const bool f32 = kind == kSpotrf;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  size_t total = 0, total_b = 0;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < 1 || n[i] > MAXN) return FLA_HIP_EARG;
+    total += (size_t)n[i] * n[i];
+    total_b += nrhs ? (size_t)n[i] * std::max(0, nrhs[i]) : 0;
+  }
+  char* buf = nullptr;
+  FLA_HIP_CHECK(hipMalloc(&buf, (total + total_b) * esz + (size_t)count * (MAXN + 1) * sizeof(int) + 256));
+  std::vector<void*> Ap(count), Bp(count);
+  std::vector<int*> Pp(count);
+  std::vector<int> lda(count), ldb(count);
+  size_t off = 0, offb = total * esz;
+  int* ipiv_base = reinterpret_cast<int*>(buf + (total + total_b) * esz);
+  int* info = ipiv_base + (size_t)count * MAXN;
+  for (int i = 0; i < count; ++i) {
+    Ap[i] = buf + off;
+    lda[i] = n[i];
+    off += (size_t)n[i] * n[i] * esz;
+    Bp[i] = buf + offb;
+    ldb[i] = n[i];
+    offb += nrhs ? (size_t)n[i] * std::max(0, nrhs[i]) * esz : 0;
+    Pp[i] = ipiv_base + (size_t)i * MAXN;
+  }
+  std::vector<BatchConfig> cands;
+  for (int nb : (f32 ? std::vector<int>{32, 64} : std::vector<int>{16, 32}))
+    for (int wg = 1; wg <= 2; ++wg) {
+      BatchConfig c;
+      c.nb = nb;
+      c.wg_per_cu = wg;
+      cands.push_back(c);
+    }
+  hipEvent_t e0, e1;
+  FLA_HIP_CHECK(hipEventCreate(&e0));
+  FLA_HIP_CHECK(hipEventCreate(&e1));
+  double best = 1e300;
+  BatchConfig bestc = default_config(kind);
+  const bool save_graph = g_use_graph;
+  g_use_graph = false;
+  int st = FLA_HIP_OK;
+  for (const BatchConfig& c : cands) {
+    float ms_sum = 0.f;
+    for (int rep = 0; rep < 3 && st == FLA_HIP_OK; ++rep) {
+      for (int i = 0; i < count; ++i) {
+        const long nn = (long)n[i] * n[i];
+        hipLaunchKernelGGL(tune_fill_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                           n[i], f32 ? nullptr : static_cast<double*>(Ap[i]),
+                           f32 ? static_cast<float*>(Ap[i]) : nullptr, kind != kDgetrf ? 1 : 0,
+                           (unsigned long long)i);
+      }
+      st = stage_items(count, n, Ap.data(), lda.data(), Pp.data(), nrhs ? Bp.data() : nullptr,
+                       ldb.data(), nrhs, kind == kDgetrf);
+      if (st != FLA_HIP_OK) break;
+      FLA_HIP_CHECK(hipMemcpyAsync(g_items_dev, g_items_pin, sizeof(BatchItem) * count,
+                                   hipMemcpyHostToDevice, s));
+      FLA_HIP_CHECK(hipEventRecord(e0, s));
+      st = launch_kind(s, kind, c, count, info);
+      FLA_HIP_CHECK(hipEventRecord(e1, s));
+      FLA_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0) ms_sum += ms;  // first run warms up
+    }
+    if (st != FLA_HIP_OK) break;
+    if (ms_sum / 2 < best) {
+      best = ms_sum / 2;
+      bestc = c;
+    }
+  }
+  g_use_graph = save_graph;
+  FLA_HIP_CHECK(hipEventDestroy(e0));
+  FLA_HIP_CHECK(hipEventDestroy(e1));
+  FLA_HIP_CHECK(hipDeviceSynchronize());
+  FLA_HIP_CHECK(hipFree(buf));
+  if (st != FLA_HIP_OK) return st;
+  const Signature sig = signature_of(count, n, nrhs);
+  bool found = false;
+  for (TunedEntry& e : g_tuned[kind])
+    if (e.sig == sig) {
+      e.cfg = bestc;
+      found = true;
+    }
+  if (!found) g_tuned[kind].push_back({sig, bestc});
+  if (best_nb) *best_nb = bestc.nb;
+  if (best_wg) *best_wg = bestc.wg_per_cu;
+  if (best_ms) *best_ms = best;
+  return FLA_HIP_OK;
+}

@@ -366,7 +366,7 @@ int launch_transpose_swap(hipStream_t s, int n, double* A, long lda) {
 // Factors columns [k, k+kb) of the lower matrix (rows k..n-1): per 128-column
 // sub-block, the diagonal factor, the blocked panel solve of everything
 // below it, and the update of the panel's remaining columns.  All on stream s.
-static int chol_panel(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info) {
+int chol_panel_f64(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info) {
   for (int kk = k; kk < k + kb; kk += PB) {
     const int ib = (k + kb - kk) < PB ? (k + kb - kk) : PB;
     double* Akk = A + (long)kk + (long)kk * lda;
@@ -400,7 +400,7 @@ static int chol_lower(int n, double* A, long lda, int* dinfo) {
 
   // Panel 0.
   int kb0 = n < nb ? n : nb;
-  FLA_TRY(chol_panel(sp, n, A, lda, 0, kb0, dinfo));
+  FLA_TRY(chol_panel_f64(sp, n, A, lda, 0, kb0, dinfo));
   for (int k = 0; k < n;) {
     const int kb = (n - k) < nb ? (n - k) : nb;
     hipEvent_t panel_done = next_event();
@@ -429,7 +429,7 @@ static int chol_lower(int n, double* A, long lda, int* dinfo) {
     }
     // Next panel on the panel stream, concurrent with (b).
     FLA_HIP_CHECK(hipStreamWaitEvent(sp, next_ready, 0));
-    FLA_TRY(chol_panel(sp, n, A, lda, k2, kb2, dinfo));
+    FLA_TRY(chol_panel_f64(sp, n, A, lda, k2, kb2, dinfo));
     k = k2;
   }
   // Join: s_update waits for the last panel.

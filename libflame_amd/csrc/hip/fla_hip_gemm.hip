@@ -57,36 +57,6 @@ __device__ __forceinline__ void store_chunk(double* S, const double (&reg)[BM * 
   }
 }
 
-// Maps a linear workgroup id to a C tile.  Workgroups are dealt to the eight
-// XCDs round-robin; the id is first permuted so that each XCD receives a
-// contiguous run of tiles (which share operand panels in its own L2).
-// mode 0: column-major over a Tm x Tn grid; mode 1: lower tiles (whose rows
-// reach the diagonal) of a Tm x Tn grid, column by column.
-__device__ __forceinline__ void tile_of(int bid, int nblk, int mode, int Tm, int Tn, int rpc,
-                                        int& ti, int& tj) {
-  const int x = bid & 7;
-  const int q = nblk >> 3, rr = nblk & 7;
-  const int t = x * q + (x < rr ? x : rr) + (bid >> 3);
-  if (mode == 0) {
-    tj = t / Tm;
-    ti = t - tj * Tm;
-  } else {
-    // Column c holds the row tiles [c*rpc, Tm): offset(c) = c*Tm - rpc*c*(c-1)/2.
-    int c = 0;
-    {
-      const double a = 2.0 * Tm + rpc;
-      const double disc = a * a - 8.0 * rpc * (double)t;
-      c = (int)((a - sqrt(disc > 0.0 ? disc : 0.0)) / (2.0 * rpc));
-    }
-    if (c < 0) c = 0;
-    if (c > Tn - 1) c = Tn - 1;
-    while (c > 0 && (long)c * Tm - (long)rpc * c * (c - 1) / 2 > t) --c;
-    while (c + 1 < Tn && (long)(c + 1) * Tm - (long)rpc * (c + 1) * c / 2 <= t) ++c;
-    tj = c;
-    ti = c * rpc + (t - (c * Tm - rpc * c * (c - 1) / 2));
-  }
-}
-
 // C := alpha*op(A)*op(B) + beta*C on a BM x BN tile per workgroup of
 // WM x WN waves; each wave owns (BM/WM) x (BN/WN) = TM x TN MFMA blocks.
 template <bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
@@ -103,7 +73,7 @@ __global__ __launch_bounds__(64 * WM * WN) void dgemm_mfma_kernel(
   __shared__ double Bs[2][BK * LB];
 
   int ti, tj;
-  tile_of(blockIdx.x, gridDim.x, mode, Tm, Tn, rpc, ti, tj);
+  fla_tile_of(blockIdx.x, gridDim.x, mode, Tm, Tn, rpc, ti, tj);
   const int i0 = ti * BM, j0 = tj * BN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;

@@ -33,9 +33,11 @@ struct FlaHipCtx {
   hipStream_t s_panel = nullptr;   // panel factorizations (look-ahead)
   static constexpr int kEvents = 8;
   hipEvent_t ev[kEvents] = {};
-  // Grow-only device workspace for scratch (info words, counters, panels).
-  void* work = nullptr;
-  size_t work_bytes = 0;
+  // Grow-only device workspaces: slot 0 general scratch (dtrsm, dlaswp),
+  // slot 1 the mixed-precision solver's buffers, slot 2 its solve scratch.
+  static constexpr int kSlots = 4;
+  void* work[kSlots] = {};
+  size_t work_bytes[kSlots] = {};
   // Small persistent device scratch: [0] info word, [1..] counters.
   int* dscratch = nullptr;
   // 128 x 128 FP64 block: inverse of the current Cholesky diagonal block.
@@ -48,6 +50,7 @@ FlaHipCtx& fla_ctx();
 int fla_ensure_ready();
 // Returns a device workspace of at least `bytes` bytes (grow-only).
 int fla_workspace(size_t bytes, void** ptr);
+int fla_workspace_slot(int slot, size_t bytes, void** ptr);
 
 #define FLA_HIP_CHECK(expr)                                                    \
   do {                                                                         \
@@ -77,6 +80,36 @@ inline double lower_update_flops(int m, int n, int k) {
   return 2.0 * k * ((mm - nn) * nn + nn * (nn + 1.0) / 2.0);
 }
 
+// Maps a linear workgroup id to a C tile.  Workgroups are dealt to the eight
+// XCDs round-robin; the id is first permuted so that each XCD receives a
+// contiguous run of tiles (which share operand panels in its own L2).
+// mode 0: column-major over a Tm x Tn grid; mode 1: lower tiles (whose rows
+// reach the diagonal) of a Tm x Tn grid, column by column.
+__device__ __forceinline__ void fla_tile_of(int bid, int nblk, int mode, int Tm, int Tn, int rpc,
+                                        int& ti, int& tj) {
+  const int x = bid & 7;
+  const int q = nblk >> 3, rr = nblk & 7;
+  const int t = x * q + (x < rr ? x : rr) + (bid >> 3);
+  if (mode == 0) {
+    tj = t / Tm;
+    ti = t - tj * Tm;
+  } else {
+    // Column c holds the row tiles [c*rpc, Tm): offset(c) = c*Tm - rpc*c*(c-1)/2.
+    int c = 0;
+    {
+      const double a = 2.0 * Tm + rpc;
+      const double disc = a * a - 8.0 * rpc * (double)t;
+      c = (int)((a - sqrt(disc > 0.0 ? disc : 0.0)) / (2.0 * rpc));
+    }
+    if (c < 0) c = 0;
+    if (c > Tn - 1) c = Tn - 1;
+    while (c > 0 && (long)c * Tm - (long)rpc * c * (c - 1) / 2 > t) --c;
+    while (c + 1 < Tn && (long)(c + 1) * Tm - (long)rpc * (c + 1) * c / 2 <= t) ++c;
+    tj = c;
+    ti = c * rpc + (t - (c * Tm - rpc * c * (c - 1) / 2));
+  }
+}
+
 // ---- kernel launchers implemented in the other translation units ----------
 // MFMA FP64 update: C := alpha*op(A)*op(B) + beta*C (mode 0 full, 1 lower).
 int launch_dgemm_mfma(hipStream_t s, bool ta, bool tb, int mode, int m, int n, int k,
@@ -99,5 +132,12 @@ constexpr int kNarrowRhs = 32;
 int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, int nrhs,
                             const double* T, long ldt, const double* W, double* X, long ldx,
                             int* done, int* err);
+// Inverts the 128x128 diagonal blocks of a triangle (lower or upper storage)
+// into W, block k at W + k*128*128 (ld 128, A's orientation, other triangle 0).
+int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool upper, bool unit,
+                        double* W);
+// FP64 factor of columns [k, k+kb) of a lower matrix (rows k..n-1): diagonal
+// factors, panel solves and in-panel updates, all on stream s.
+int chol_panel_f64(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info);
 // In-place exchange of the strictly lower and strictly upper triangles.
 int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

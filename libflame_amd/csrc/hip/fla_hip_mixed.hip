@@ -1,0 +1,852 @@
+// Mixed-precision Cholesky solve with iterative refinement: the semantics of
+// dsposv_ (src/map/common/lapacksrc/f2c/dsposv.c:192-415) -- FP32 working
+// copy of A, refinement with an FP64 residual until
+//   max|r_j| <= max|x_j| * ||A||_inf * eps * sqrt(n)   for every column j
+// (cte :297, tests :340 and :386), at most 30 refinement steps (:351), and a
+// fall-back to the FP64 factorization and solve when the conversion
+// overflows (iter = -2), the reduced-precision factorization breaks down
+// (iter = -3) or refinement does not converge (iter = -31), with A
+// overwritten by the FP64 factor in that case (:403-405).
+//
+// The reduced-precision factor (fla_hip_mp_chol) keeps the trailing matrix
+// in FP32 and runs its updates on MFMA in FP32 or BF16 with FP32
+// accumulation; each panel is factored in FP64 (diagonal blocks and panel
+// solve) from the FP32 trailing values, and the FP64 panel is kept as the
+// factor used by the refinement solves.  The residual r = b - A x and the
+// update of x are FP64.  Besides classic refinement (x += M^-1 r with
+// M = L L^T), GMRES-IR solves A d = r by left-preconditioned GMRES in FP64
+// with M^-1 as the preconditioner, which converges where kappa(A) * u_factor
+// is too large for classic refinement.
+#include "fla_hip_internal.h"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+// Launchers of fla_hip_lpgemm.hip.
+long lp_panel_elems(int m, int k, bool bf16);
+int launch_lp_absmax(hipStream_t s, int m, int k, const double* P, long ldp, unsigned* out);
+int launch_lp_pack(hipStream_t s, bool bf16, int m, int k, const double* P, long ldp,
+                   const unsigned* amax_bits, void* out);
+int launch_lp_update(hipStream_t s, bool bf16, bool lower, int m, int n, int k, const void* P,
+                     int prow0, const void* Q, int qrow0, float alpha, const unsigned* amax_bits,
+                     float* C, long ldc, const int* abort_flag);
+
+namespace {
+
+constexpr int NB = 128;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---- conversions ----------------------------------------------------------
+
+// As := float(A) on the lower triangle of the symmetric matrix given by its
+// `upper` or lower triangle (32 x 32 tiles through LDS, so the transposed
+// read of upper storage stays coalesced).  *ovf = 1 when an entry lies
+// outside [-FLT_MAX, FLT_MAX] (dlat2s_).
+__global__ void lat2s_kernel(int n, const double* __restrict__ A, long lda, int upper,
+                             float* __restrict__ As, long ldas, int* __restrict__ ovf) {
+  __shared__ double tile[32][33];
+  // lower tile (bi >= bj) number blockIdx.x, column by column
+  const int T = (n + 31) / 32;
+  int bj = 0, rem = blockIdx.x;
+  while (rem >= T - bj) {
+    rem -= T - bj;
+    ++bj;
+  }
+  const int bi = bj + rem;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  bool bad = false;
+  if (!upper) {
+    for (int r = ty; r < 32; r += 8) {
+      const int i = bi * 32 + tx, j = bj * 32 + r;
+      if (i < n && j < n && i >= j) {
+        const double v = A[(long)i + (long)j * lda];
+        bad |= (v < -FLT_MAX || v > FLT_MAX);
+        As[(long)i + (long)j * ldas] = (float)v;
+      }
+    }
+  } else {
+    // element (i, j), i >= j, of the symmetric matrix is A(j, i) of the upper triangle
+    for (int r = ty; r < 32; r += 8) {
+      const int jj = bj * 32 + tx, ii = bi * 32 + r;  // read A(jj, ii), contiguous in jj
+      tile[r][tx] = (jj < n && ii < n && jj <= ii) ? A[(long)jj + (long)ii * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      const int i = bi * 32 + tx, j = bj * 32 + r;
+      if (i < n && j < n && i >= j) {
+        const double v = tile[tx][r];
+        bad |= (v < -FLT_MAX || v > FLT_MAX);
+        As[(long)i + (long)j * ldas] = (float)v;
+      }
+    }
+  }
+  if (bad) atomicOr(ovf, 1);
+}
+
+// Ld := double(As) on the lower part of an m x kb panel whose top-left
+// element is on the diagonal; resets the panel's absmax slot.
+__global__ void s2d_panel_kernel(int m, int kb, const float* __restrict__ As, long ldas,
+                                 double* __restrict__ Ld, long ldl, unsigned* amax) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx == 0) *amax = 0u;
+  if (idx >= (long)m * kb) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  if (i >= j) Ld[(long)i + (long)j * ldl] = (double)As[(long)i + (long)j * ldas];
+}
+
+// ---- symmetric products and norms -------------------------------------------
+
+// Element (i, j) of the symmetric matrix stored in the lower or upper triangle.
+__device__ __forceinline__ double sym_at(const double* A, long lda, int upper, int i, int j) {
+  const bool direct = upper ? (i <= j) : (i >= j);
+  return direct ? A[(long)i + (long)j * lda] : A[(long)j + (long)i * lda];
+}
+
+// Partial products of R := A X for a symmetric A (block row I, blocks
+// J in [J0, J1)): Rp[(split)*n*16 + i + col*n].  4 waves x (32 rows x 16
+// columns) of v_mfma_f64_16x16x4f64 per 128-row block, X staged through LDS
+// in fragment order, A fragments read directly (transposed for the blocks
+// taken from the other triangle).
+__global__ __launch_bounds__(256) void symm_partial_kernel(int n, int nrhs,
+                                                           const double* __restrict__ A, long lda,
+                                                           int upper, const double* __restrict__ X,
+                                                           long ldx, int jchunk,
+                                                           double* __restrict__ Rp) {
+  __shared__ double Xf[NB * 16];
+  const int nb = (n + NB - 1) / NB;
+  const int I = blockIdx.x, split = blockIdx.y;
+  const int J0 = split * jchunk, J1 = (J0 + jchunk) < nb ? (J0 + jchunk) : nb;
+  const int r0 = I * NB;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+  d4 acc[2];
+  acc[0] = d4{0.0, 0.0, 0.0, 0.0};
+  acc[1] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int J = J0; J < J1; ++J) {
+    const int c0 = J * NB;
+    __syncthreads();
+    for (int e = t; e < NB * 16; e += 256) {
+      const int k = e & (NB - 1), col = e >> 7;
+      const double v = (c0 + k < n && col < nrhs) ? X[(long)(c0 + k) + (long)col * ldx] : 0.0;
+      Xf[(k & 31) * 64 + (k >> 5) * 16 + col] = v;
+    }
+    __syncthreads();
+    // direct block: stored triangle holds (I, J); mirrored: it holds (J, I)
+    const int mode = (I == J) ? 2 : ((upper ? (I < J) : (I > J)) ? 0 : 1);
+    const bool full = (r0 + NB <= n) && (c0 + NB <= n);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int row = r0 + 32 * w + 16 * a + li;
+      double af[32];
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks) {
+        const int k = c0 + 32 * lk + ks;
+        double v = 0.0;
+        if (full || (row < n && k < n)) {
+          if (mode == 0)
+            v = A[(long)row + (long)k * lda];
+          else if (mode == 1)
+            v = A[(long)k + (long)row * lda];
+          else
+            v = sym_at(A, lda, upper, row, k);
+        }
+        af[ks] = v;
+      }
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks)
+        acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], af[ks], acc[a], 0, 0, 0);
+    }
+  }
+  double* R = Rp + (long)split * n * 16;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int row = r0 + 32 * w + 16 * a + li;
+    if (row >= n) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = lk + 4 * r;
+      if (col < nrhs) R[(long)row + (long)col * n] = acc[a][r];
+    }
+  }
+}
+
+// R := B - sum_split Rp, or R := sum_split Rp when B is null (fixed
+// summation order: deterministic).
+__global__ void symm_finish_kernel(int n, int nrhs, const double* __restrict__ B, long ldb,
+                                   const double* __restrict__ Rp, int nsplit,
+                                   double* __restrict__ R, long ldr) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int i = (int)(idx % n), c = (int)(idx / n);
+  double s = 0.0;
+  for (int p = 0; p < nsplit; ++p) s += Rp[(long)p * n * 16 + i + (long)c * n];
+  R[(long)i + (long)c * ldr] = B ? B[(long)i + (long)c * ldb] - s : s;
+}
+
+// Partial row sums of |A| for the infinity norm of a symmetric matrix: tile
+// (bi, bj) of the stored triangle (64 x 64, bi >= bj in lower orientation)
+// adds its row sums to Prow[bj][bi*64 + r] and, off the diagonal, its column
+// sums (the mirrored rows) to Pcol[bi][bj*64 + c].
+__global__ void syminf_partial_kernel(int n, const double* __restrict__ A, long lda, int upper,
+                                      double* __restrict__ Prow, double* __restrict__ Pcol) {
+  __shared__ double tile[64][65];
+  const int T = (n + 63) / 64;
+  int bj = 0, rem = blockIdx.x;
+  while (rem >= T - bj) {
+    rem -= T - bj;
+    ++bj;
+  }
+  const int bi = bj + rem;
+  const int t = threadIdx.x;
+  // tile[r][c] = |S(bi*64 + r, bj*64 + c)| (0 outside the matrix)
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    const int i = bi * 64 + r, j = bj * 64 + c;
+    double v = 0.0;
+    if (i < n && j < n && (bi != bj || i >= j)) v = fabs(sym_at(A, lda, upper, i, j));
+    tile[r][c] = v;
+  }
+  __syncthreads();
+  if (t < 64) {
+    double s = 0.0;
+    for (int c = 0; c < 64; ++c) s += tile[t][c];
+    Prow[(long)bj * n + bi * 64 + t] = (bi * 64 + t < n) ? s : 0.0;
+  } else if (t < 128) {
+    const int c = t - 64;
+    double s = 0.0;
+    for (int r = 0; r < 64; ++r) s += (bi == bj && r == c) ? 0.0 : tile[r][c];
+    Pcol[(long)bi * n + bj * 64 + c] = (bj * 64 + c < n) ? s : 0.0;
+  }
+}
+
+// out[0] = max_i (sum_{bj <= bi(i)} Prow[bj][i] + sum_{bi >= bj(i)} Pcol[bi][i]).
+__global__ void syminf_finish_kernel(int n, const double* __restrict__ Prow,
+                                     const double* __restrict__ Pcol, double* out) {
+  __shared__ double red[256];
+  const int T = (n + 63) / 64;
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int b = i / 64;
+    double s = 0.0;
+    for (int bj = 0; bj <= b; ++bj) s += Prow[(long)bj * n + i];
+    for (int bi = b; bi < T; ++bi) s += Pcol[(long)bi * n + i];
+    mx = (s > mx || s != s) ? s : mx;
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// out[c] = max_i |V(i, c)| (one workgroup per column); NaN propagates.
+__global__ void colmax_kernel(int n, const double* __restrict__ V, long ldv,
+                              double* __restrict__ out) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double a = fabs(V[(long)i + (long)c * ldv]);
+    mx = (a > mx || a != a) ? a : mx;
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// Y(:, c) += alpha * X(:, c) on n x nrhs blocks.
+__global__ void axpy_block_kernel(int n, int nrhs, double alpha, const double* __restrict__ X,
+                                  long ldx, double* __restrict__ Y, long ldy) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int i = (int)(idx % n), c = (int)(idx / n);
+  Y[(long)i + (long)c * ldy] += alpha * X[(long)i + (long)c * ldx];
+}
+
+// ---- GMRES kernels (columns independent, Krylov block j at V + j*n*nrhs) ---
+
+// H[i*nrhs + c] = V_i(:, c) . w(:, c) for i < j (one workgroup per (i, c)).
+__global__ void block_dots_kernel(int n, int nrhs, const double* __restrict__ V,
+                                  const double* __restrict__ w, double* __restrict__ H) {
+  __shared__ double red[256];
+  const int i = blockIdx.x, c = blockIdx.y;
+  const double* v = V + (long)i * n * nrhs + (long)c * n;
+  const double* x = w + (long)c * n;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) s += v[k] * x[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) H[(long)i * nrhs + c] = red[0];
+}
+
+// w(:, c) -= sum_{i<j} H[i*nrhs + c] * V_i(:, c)
+__global__ void block_sub_kernel(int n, int nrhs, int j, const double* __restrict__ V,
+                                 const double* __restrict__ H, double* __restrict__ w) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int k = (int)(idx % n), c = (int)(idx / n);
+  double s = w[idx];
+  for (int i = 0; i < j; ++i) s -= H[(long)i * nrhs + c] * V[(long)i * n * nrhs + (long)c * n + k];
+  w[idx] = s;
+}
+
+// out(:, c) = sum_{i<j} y[i*nrhs + c] * V_i(:, c)  (+ out if accumulate)
+__global__ void block_combine_kernel(int n, int nrhs, int j, const double* __restrict__ V,
+                                     const double* __restrict__ y, double* __restrict__ out,
+                                     long ldo, int accumulate) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int k = (int)(idx % n), c = (int)(idx / n);
+  double s = accumulate ? out[(long)k + (long)c * ldo] : 0.0;
+  for (int i = 0; i < j; ++i) s += y[(long)i * nrhs + c] * V[(long)i * n * nrhs + (long)c * n + k];
+  out[(long)k + (long)c * ldo] = s;
+}
+
+// v(:, c) *= scale[c]
+__global__ void block_scale_kernel(int n, int nrhs, const double* __restrict__ scale,
+                                   double* __restrict__ v) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int c = (int)(idx / n);
+  v[idx] *= scale[c];
+}
+
+inline unsigned nblocks(long total) { return (unsigned)((total + 255) / 256); }
+
+
+// ---- reduced-precision factorization ------------------------------------------
+
+struct LpFactor {
+  float* As = nullptr;      // FP32 trailing matrix (lower), ld n
+  void* P[2] = {};          // packed panels (double-buffered across steps)
+  unsigned* amax = nullptr; // two absmax slots
+};
+
+// Panel step: L[k:n, k:k+kb] := FP64 factor of the FP32 trailing values, and
+// the rows below the diagonal block packed (scaled) into P.
+int lp_panel(hipStream_t s, int n, LpFactor& f, double* L, long ldl, int k, int kb, bool bf16,
+             int slot, int* dinfo) {
+  const int m = n - k;
+  hipLaunchKernelGGL(s2d_panel_kernel, dim3(nblocks((long)m * kb)), dim3(256), 0, s, m, kb,
+                     f.As + (long)k + (long)k * n, (long)n, L + (long)k + (long)k * ldl, ldl,
+                     f.amax + slot);
+  FLA_HIP_CHECK(hipGetLastError());
+  FLA_TRY(chol_panel_f64(s, n, L, ldl, k, kb, dinfo));
+  const int m2 = m - kb;
+  if (m2 > 0) {
+    const double* L21 = L + (long)(k + kb) + (long)k * ldl;
+    FLA_TRY(launch_lp_absmax(s, m2, kb, L21, ldl, f.amax + slot));
+    FLA_TRY(launch_lp_pack(s, bf16, m2, kb, L21, ldl, f.amax + slot, f.P[slot]));
+  }
+  return FLA_HIP_OK;
+}
+
+// Right-looking blocked factor with look-ahead: the panel of step k+1 (FP64,
+// s_panel) overlaps the reduced-precision update of step k (s_update).
+int lp_chol_lower(int n, LpFactor& f, double* L, long ldl, bool bf16, int nb, int* dinfo) {
+  FlaHipCtx& c = fla_ctx();
+  hipStream_t sp = c.s_panel, su = c.s_update;
+  int evi = 0;
+  auto next_event = [&]() {
+    hipEvent_t e = c.ev[evi];
+    evi = (evi + 1) % FlaHipCtx::kEvents;
+    return e;
+  };
+  hipEvent_t start = next_event();
+  FLA_HIP_CHECK(hipEventRecord(start, su));
+  FLA_HIP_CHECK(hipStreamWaitEvent(sp, start, 0));
+  int slot = 0;
+  FLA_TRY(lp_panel(sp, n, f, L, ldl, 0, n < nb ? n : nb, bf16, slot, dinfo));
+  for (int k = 0; k < n;) {
+    const int kb = (n - k) < nb ? (n - k) : nb;
+    hipEvent_t panel_done = next_event();
+    FLA_HIP_CHECK(hipEventRecord(panel_done, sp));
+    FLA_HIP_CHECK(hipStreamWaitEvent(su, panel_done, 0));
+    const int k2 = k + kb, m2 = n - k2;
+    if (m2 <= 0) break;
+    const int kb2 = m2 < nb ? m2 : nb;
+    float* C2 = f.As + (long)k2 + (long)k2 * n;
+    void* tok = fla_stats_begin(su);
+    FLA_TRY(launch_lp_update(su, bf16, true, m2, kb2, kb, f.P[slot], 0, f.P[slot], 0, -1.0f,
+                             f.amax + slot, C2, n, dinfo));
+    fla_stats_end(su, tok, 4, lower_update_flops(m2, kb2, kb));
+    hipEvent_t next_ready = next_event();
+    FLA_HIP_CHECK(hipEventRecord(next_ready, su));
+    const int m3 = m2 - kb2;
+    if (m3 > 0) {
+      tok = fla_stats_begin(su);
+      FLA_TRY(launch_lp_update(su, bf16, true, m3, m3, kb, f.P[slot], kb2, f.P[slot], kb2, -1.0f,
+                               f.amax + slot, C2 + (long)kb2 + (long)kb2 * n, n, dinfo));
+      fla_stats_end(su, tok, 4, lower_update_flops(m3, m3, kb));
+    }
+    FLA_HIP_CHECK(hipStreamWaitEvent(sp, next_ready, 0));
+    FLA_TRY(lp_panel(sp, n, f, L, ldl, k2, kb2, bf16, slot ^ 1, dinfo));
+    slot ^= 1;
+    k = k2;
+  }
+  hipEvent_t fin = next_event();
+  FLA_HIP_CHECK(hipEventRecord(fin, sp));
+  FLA_HIP_CHECK(hipStreamWaitEvent(su, fin, 0));
+  return FLA_HIP_OK;
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+// Factor of the symmetric A (uplo triangle) in reduced precision; lower FP64
+// factor into L.  *info: kSuccess, or the 0-based failing column; *ovf = 1
+// when A does not fit in FP32 (nothing factored then).
+int mp_chol(int upper, int n, const double* A, long lda, bool bf16, double* L, long ldl,
+            int* info, int* ovf) {
+  FlaHipCtx& c = fla_ctx();
+  hipStream_t s = c.s_update;
+  const int nb = c.chol_nb;
+  const size_t as_bytes = ((size_t)n * n * sizeof(float) + 255) & ~(size_t)255;
+  const size_t p_bytes = ((size_t)lp_panel_elems(n, nb, bf16) * (bf16 ? 2 : 4) + 255) &
+                         ~(size_t)255;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace_slot(1, as_bytes + 2 * p_bytes + 256, &w));
+  LpFactor f;
+  char* base = static_cast<char*>(w);
+  f.As = reinterpret_cast<float*>(base);
+  f.P[0] = base + as_bytes;
+  f.P[1] = base + as_bytes + p_bytes;
+  f.amax = reinterpret_cast<unsigned*>(base + as_bytes + 2 * p_bytes);
+  int* dinfo = c.dscratch;      // [0] info
+  int* dovf = c.dscratch + 12;  // overflow flag
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, dinfo, (int)kSuccess);
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, dovf, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  const int T = (n + 31) / 32;
+  hipLaunchKernelGGL(lat2s_kernel, dim3((unsigned)((long)T * (T + 1) / 2)), dim3(256), 0, s, n, A,
+                     lda, upper, f.As, (long)n, dovf);
+  FLA_HIP_CHECK(hipGetLastError());
+  int h[2] = {0, 0};
+  FLA_HIP_CHECK(hipMemcpyAsync(h, dovf, sizeof(int), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  *ovf = h[0];
+  *info = kSuccess;
+  if (h[0]) return FLA_HIP_OK;
+  FLA_TRY(lp_chol_lower(n, f, L, ldl, bf16, nb, dinfo));
+  FLA_HIP_CHECK(hipMemcpyAsync(h, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  fla_stats_harvest();
+  *info = h[0];
+  return FLA_HIP_OK;
+}
+
+// ---- refinement ----------------------------------------------------------------
+
+struct MpState {
+  hipStream_t s = nullptr;
+  int n = 0, nrhs = 0, upper = 0;
+  const double* A = nullptr;
+  long lda = 0;
+  double* L = nullptr;   // FP64-valued reduced-precision factor (lower), ld n
+  double* W = nullptr;   // inverted 128-blocks of L
+  int* flags = nullptr;  // solve counters
+  double* Rp = nullptr;  // symm partials
+  int nsplit = 1, jchunk = 1;
+  double* hbuf = nullptr;  // device scratch for norms / GMRES coefficients
+  int solves = 0, matvecs = 0;
+};
+
+// X := (L L^T)^-1 X on an n x nrhs block.
+int mp_apply_prec(MpState& st, double* X, long ldx) {
+  FLA_TRY(launch_trsm_left_narrow(st.s, true, false, st.n, st.nrhs, st.L, st.n, st.W, X, ldx,
+                                  st.flags, st.flags + 1));
+  FLA_TRY(launch_trsm_left_narrow(st.s, false, true, st.n, st.nrhs, st.L, st.n, st.W, X, ldx,
+                                  st.flags, st.flags + 1));
+  ++st.solves;
+  return FLA_HIP_OK;
+}
+
+// R := B - A X, or R := A X when B is null (FP64, symmetric A).
+int mp_residual(MpState& st, const double* B, long ldb, const double* X, long ldx, double* R,
+                long ldr) {
+  const int nb = (st.n + NB - 1) / NB;
+  for (int c = 0; c < st.nrhs; c += 16) {
+    const int nr = (st.nrhs - c) < 16 ? (st.nrhs - c) : 16;
+    hipLaunchKernelGGL(symm_partial_kernel, dim3(nb, st.nsplit), dim3(256), 0, st.s, st.n, nr,
+                       st.A, st.lda, st.upper, X + (long)c * ldx, ldx, st.jchunk, st.Rp);
+    FLA_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(symm_finish_kernel, dim3(nblocks((long)st.n * nr)), dim3(256), 0, st.s,
+                       st.n, nr, B ? B + (long)c * ldb : nullptr, ldb, st.Rp, st.nsplit,
+                       R + (long)c * ldr, ldr);
+    FLA_HIP_CHECK(hipGetLastError());
+  }
+  ++st.matvecs;
+  return FLA_HIP_OK;
+}
+
+int mp_colmax(MpState& st, const double* V, long ldv, double* host) {
+  hipLaunchKernelGGL(colmax_kernel, dim3(st.nrhs), dim3(256), 0, st.s, st.n, V, ldv, st.hbuf);
+  FLA_HIP_CHECK(hipGetLastError());
+  FLA_HIP_CHECK(hipMemcpyAsync(host, st.hbuf, sizeof(double) * st.nrhs, hipMemcpyDeviceToHost,
+                               st.s));
+  FLA_HIP_CHECK(hipStreamSynchronize(st.s));
+  return FLA_HIP_OK;
+}
+
+// dsposv's test: every column has max|r| <= max|x| * cte (dsposv.c:336-344).
+int mp_converged(MpState& st, const double* X, const double* R, double cte, bool* ok,
+                 bool* r_overflow) {
+  std::vector<double> xn(st.nrhs), rn(st.nrhs);
+  FLA_TRY(mp_colmax(st, X, st.n, xn.data()));
+  FLA_TRY(mp_colmax(st, R, st.n, rn.data()));
+  *ok = true;
+  *r_overflow = false;
+  for (int c = 0; c < st.nrhs; ++c) {
+    if (rn[c] > xn[c] * cte || rn[c] != rn[c]) *ok = false;
+    if (rn[c] > FLT_MAX) *r_overflow = true;
+  }
+  return FLA_HIP_OK;
+}
+
+// H[i*q + c] = V_i(:,c) . w(:,c) for i < j, into host h.
+int mp_dots(MpState& st, const double* V, int j, const double* w, double* h) {
+  hipLaunchKernelGGL(block_dots_kernel, dim3(j, st.nrhs), dim3(256), 0, st.s, st.n, st.nrhs, V,
+                     w, st.hbuf);
+  FLA_HIP_CHECK(hipGetLastError());
+  FLA_HIP_CHECK(hipMemcpyAsync(h, st.hbuf, sizeof(double) * j * st.nrhs, hipMemcpyDeviceToHost,
+                               st.s));
+  FLA_HIP_CHECK(hipStreamSynchronize(st.s));
+  return FLA_HIP_OK;
+}
+
+int mp_upload(MpState& st, const double* h, int count) {
+  FLA_HIP_CHECK(hipMemcpyAsync(st.hbuf, h, sizeof(double) * count, hipMemcpyHostToDevice, st.s));
+  FLA_HIP_CHECK(hipStreamSynchronize(st.s));
+  return FLA_HIP_OK;
+}
+
+// D := approximation of A^-1 R by GMRES(m) on M^-1 A d = M^-1 r (M = L L^T),
+// all columns in lockstep, each with its own Krylov basis (classical
+// Gram-Schmidt with one re-orthogonalization).  A column stops when its
+// preconditioned residual is below tol times its initial one.
+int mp_gmres(MpState& st, const double* R, double* D, double* V, double* Wv, int m, double tol,
+             int* iters) {
+  const int n = st.n, q = st.nrhs;
+  const long blk = (long)n * q;
+  const unsigned g = nblocks(blk);
+  FLA_HIP_CHECK(hipMemcpyAsync(V, R, sizeof(double) * blk, hipMemcpyDeviceToDevice, st.s));
+  FLA_TRY(mp_apply_prec(st, V, n));
+  std::vector<double> beta(q), sc(q);
+  FLA_TRY(mp_dots(st, V, 1, V, beta.data()));
+  for (int c = 0; c < q; ++c) {
+    beta[c] = std::sqrt(beta[c]);
+    sc[c] = beta[c] > 0.0 ? 1.0 / beta[c] : 0.0;
+  }
+  FLA_TRY(mp_upload(st, sc.data(), q));
+  hipLaunchKernelGGL(block_scale_kernel, dim3(g), dim3(256), 0, st.s, n, q, st.hbuf, V);
+  FLA_HIP_CHECK(hipGetLastError());
+  const int ld = m + 1;
+  std::vector<double> H((size_t)q * ld * m, 0.0), cs((size_t)q * m, 0.0), sn((size_t)q * m, 0.0),
+      gv((size_t)q * ld, 0.0), h((size_t)ld * q), h2((size_t)ld * q);
+  std::vector<int> kc(q, 0);
+  std::vector<char> done(q, 0);
+  for (int c = 0; c < q; ++c) {
+    gv[(size_t)c * ld] = beta[c];
+    done[c] = !(beta[c] > 0.0);
+  }
+  int j = 0;
+  for (j = 0; j < m; ++j) {
+    bool all = true;
+    for (int c = 0; c < q; ++c) all = all && done[c];
+    if (all) break;
+    const double* Vj = V + (long)j * blk;
+    double* Vn = V + (long)(j + 1) * blk;
+    FLA_TRY(mp_residual(st, nullptr, n, Vj, n, Wv, n));  // Wv = A V_j
+    FLA_TRY(mp_apply_prec(st, Wv, n));                    // Wv = M^-1 A V_j
+    // CGS with one re-orthogonalization
+    FLA_TRY(mp_dots(st, V, j + 1, Wv, h.data()));
+    FLA_TRY(mp_upload(st, h.data(), (j + 1) * q));
+    hipLaunchKernelGGL(block_sub_kernel, dim3(g), dim3(256), 0, st.s, n, q, j + 1, V, st.hbuf, Wv);
+    FLA_HIP_CHECK(hipGetLastError());
+    FLA_TRY(mp_dots(st, V, j + 1, Wv, h2.data()));
+    FLA_TRY(mp_upload(st, h2.data(), (j + 1) * q));
+    hipLaunchKernelGGL(block_sub_kernel, dim3(g), dim3(256), 0, st.s, n, q, j + 1, V, st.hbuf, Wv);
+    FLA_HIP_CHECK(hipGetLastError());
+    std::vector<double> nrm(q);
+    FLA_TRY(mp_dots(st, Wv, 1, Wv, nrm.data()));
+    for (int c = 0; c < q; ++c) {
+      nrm[c] = std::sqrt(nrm[c]);
+      sc[c] = (nrm[c] > 0.0 && !done[c]) ? 1.0 / nrm[c] : 0.0;
+    }
+    FLA_TRY(mp_upload(st, sc.data(), q));
+    FLA_HIP_CHECK(hipMemcpyAsync(Vn, Wv, sizeof(double) * blk, hipMemcpyDeviceToDevice, st.s));
+    hipLaunchKernelGGL(block_scale_kernel, dim3(g), dim3(256), 0, st.s, n, q, st.hbuf, Vn);
+    FLA_HIP_CHECK(hipGetLastError());
+    for (int c = 0; c < q; ++c) {
+      if (done[c]) continue;
+      double* Hc = &H[(size_t)c * ld * m + (size_t)j * ld];
+      for (int i = 0; i <= j; ++i) Hc[i] = h[(size_t)i * q + c] + h2[(size_t)i * q + c];
+      Hc[j + 1] = nrm[c];
+      for (int i = 0; i < j; ++i) {  // previous rotations
+        const double a = Hc[i], b = Hc[i + 1];
+        Hc[i] = cs[(size_t)c * m + i] * a + sn[(size_t)c * m + i] * b;
+        Hc[i + 1] = -sn[(size_t)c * m + i] * a + cs[(size_t)c * m + i] * b;
+      }
+      const double a = Hc[j], b = Hc[j + 1];
+      const double r = std::hypot(a, b);
+      const double cj = r > 0.0 ? a / r : 1.0, sj = r > 0.0 ? b / r : 0.0;
+      cs[(size_t)c * m + j] = cj;
+      sn[(size_t)c * m + j] = sj;
+      Hc[j] = r;
+      Hc[j + 1] = 0.0;
+      double* gc = &gv[(size_t)c * ld];
+      gc[j + 1] = -sj * gc[j];
+      gc[j] = cj * gc[j];
+      kc[c] = j + 1;
+      if (std::fabs(gc[j + 1]) <= tol * beta[c] || !(nrm[c] > 0.0)) done[c] = 1;
+    }
+  }
+  *iters = j;
+  // y = H^-1 g per column (upper triangular, size kc), D = V y
+  std::vector<double> y((size_t)(m + 1) * q, 0.0);
+  for (int c = 0; c < q; ++c) {
+    const int k = kc[c];
+    const double* Hc = &H[(size_t)c * ld * m];
+    const double* gc = &gv[(size_t)c * ld];
+    for (int i = k - 1; i >= 0; --i) {
+      double s = gc[i];
+      for (int l = i + 1; l < k; ++l) s -= Hc[(size_t)l * ld + i] * y[(size_t)l * q + c];
+      y[(size_t)i * q + c] = Hc[(size_t)i * ld + i] != 0.0 ? s / Hc[(size_t)i * ld + i] : 0.0;
+    }
+  }
+  int jmax = 0;
+  for (int c = 0; c < q; ++c) jmax = kc[c] > jmax ? kc[c] : jmax;
+  FLA_TRY(mp_upload(st, y.data(), jmax * q));
+  hipLaunchKernelGGL(block_combine_kernel, dim3(g), dim3(256), 0, st.s, n, q, jmax, V, st.hbuf, D,
+                     (long)n, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+}  // namespace
+
+// ---- entry points ------------------------------------------------------------------
+
+static double g_mp_stats[8];
+
+extern "C" int fla_hip_mp_chol(int uplo, int n, const double* A, int lda, int precision,
+                               double* L, int ldl, int* info) {
+  FLA_TRY(fla_ensure_ready());
+  if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
+  if (precision != FLA_MP_FP32 && precision != FLA_MP_BF16) return FLA_HIP_EARG;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldl < (n > 1 ? n : 1)) return FLA_HIP_EARG;
+  *info = kSuccess;
+  if (n == 0) return FLA_HIP_OK;
+  int ovf = 0;
+  FLA_TRY(mp_chol(uplo == kUpper, n, A, lda, precision == FLA_MP_BF16, L, ldl, info, &ovf));
+  if (ovf) return FLA_HIP_ERANGE;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* B,
+                              int ldb, double* X, int ldx, const fla_hip_mp_opts* opts, int* iter,
+                              int* info) {
+  FLA_TRY(fla_ensure_ready());
+  fla_hip_mp_opts o;
+  o.precision = FLA_MP_FP32;
+  o.refinement = FLA_IR_CLASSIC;
+  o.max_iter = 30;
+  o.gmres_restart = 20;
+  o.gmres_tol = 1e-4;
+  if (opts) o = *opts;
+  if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
+  if (o.precision != FLA_MP_FP32 && o.precision != FLA_MP_BF16) return FLA_HIP_EARG;
+  if (o.refinement != FLA_IR_CLASSIC && o.refinement != FLA_IR_GMRES) return FLA_HIP_EARG;
+  if (n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1) ||
+      ldx < (n > 1 ? n : 1) || o.max_iter < 0 || o.gmres_restart < 1)
+    return FLA_HIP_EARG;
+  *iter = 0;
+  *info = 0;
+  for (double& v : g_mp_stats) v = 0.0;
+  if (n == 0 || nrhs == 0) return FLA_HIP_OK;
+  FlaHipCtx& c = fla_ctx();
+  hipStream_t s = c.s_update;
+  hipEvent_t e0, e1, e2;
+  FLA_HIP_CHECK(hipEventCreate(&e0));
+  FLA_HIP_CHECK(hipEventCreate(&e1));
+  FLA_HIP_CHECK(hipEventCreate(&e2));
+  FLA_HIP_CHECK(hipEventRecord(e0, s));
+
+  MpState st;
+  st.s = s;
+  st.n = n;
+  st.nrhs = nrhs;
+  st.upper = (uplo == kUpper);
+  st.A = A;
+  st.lda = lda;
+  const int nb = (n + NB - 1) / NB;
+  st.nsplit = nb < 8 ? nb : 8;
+  st.jchunk = (nb + st.nsplit - 1) / st.nsplit;
+  const int gm = o.refinement == FLA_IR_GMRES ? o.gmres_restart : 0;
+  const long blk = (long)n * nrhs;
+  // slot 2: L | W | Rp | Xw R D Wv | V (gm+1 blocks) | hbuf | flags
+  const size_t szL = (size_t)n * n, szW = (size_t)nb * NB * NB, szRp = (size_t)st.nsplit * n * 16;
+  const int T64 = (n + 63) / 64;
+  size_t szV = (size_t)(gm + 1) * blk;
+  if (szV < (size_t)2 * T64 * n) szV = (size_t)2 * T64 * n;  // norm partials share it
+  const size_t szH = (size_t)4 * (gm + 2) * nrhs + 64;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace_slot(2, sizeof(double) * (szL + szW + szRp + 4 * blk + szV + szH) + 256,
+                             &w));
+  double* p = static_cast<double*>(w);
+  st.L = p;
+  p += szL;
+  st.W = p;
+  p += szW;
+  st.Rp = p;
+  p += szRp;
+  double* R = p;
+  p += blk;
+  double* D = p;
+  p += blk;
+  double* Wv = p;
+  p += blk;
+  double* Bd = p;  // dense copy of B (ld n)
+  p += blk;
+  double* V = p;
+  p += szV;
+  st.hbuf = p;
+  p += szH;
+  st.flags = reinterpret_cast<int*>(p);
+
+  FLA_HIP_CHECK(hipMemcpy2DAsync(Bd, sizeof(double) * n, B, sizeof(double) * ldb,
+                                 sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
+  // ||A||_inf and cte (dsposv.c:295-297)
+  double* Prow = V;  // the Krylov space is free before refinement
+  double* Pcol = Prow + (long)T64 * n;
+  hipLaunchKernelGGL(syminf_partial_kernel, dim3((unsigned)((long)T64 * (T64 + 1) / 2)),
+                     dim3(256), 0, s, n, A, (long)lda, st.upper, Prow, Pcol);
+  hipLaunchKernelGGL(syminf_finish_kernel, dim3(1), dim3(256), 0, s, n, Prow, Pcol, st.hbuf);
+  FLA_HIP_CHECK(hipGetLastError());
+  double anrm = 0.0;
+  FLA_HIP_CHECK(hipMemcpyAsync(&anrm, st.hbuf, sizeof(double), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  const double eps = DBL_EPSILON * 0.5;
+  const double cte = anrm * eps * std::sqrt((double)n);
+
+  int code = 0;  // 0 ok, -2 overflow, -3 factor failed, -31 no convergence
+  {
+    std::vector<double> bn(nrhs);
+    FLA_TRY(mp_colmax(st, Bd, n, bn.data()));
+    for (double v : bn)
+      if (v > FLT_MAX) code = -2;
+  }
+  int finfo = kSuccess;
+  if (code == 0) {
+    int ovf = 0;
+    FLA_TRY(mp_chol(st.upper, n, A, lda, o.precision == FLA_MP_BF16, st.L, n, &finfo, &ovf));
+    if (ovf)
+      code = -2;
+    else if (finfo != kSuccess)
+      code = -3;
+  }
+  FLA_HIP_CHECK(hipEventRecord(e1, s));
+  int it = 0;
+  if (code == 0) {
+    FLA_TRY(launch_trtri_blocks(s, n, st.L, n, false, false, st.W));
+    // x = M^-1 b, r = b - A x
+    FLA_HIP_CHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, Bd, sizeof(double) * n,
+                                   sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
+    FLA_TRY(mp_apply_prec(st, X, ldx));
+    FLA_TRY(mp_residual(st, Bd, n, X, ldx, R, n));
+    // mp_converged reads X with ld n: keep a dense copy in D for the test
+    bool ok = false, rovf = false;
+    auto test = [&]() -> int {
+      FLA_HIP_CHECK(hipMemcpy2DAsync(D, sizeof(double) * n, X, sizeof(double) * ldx,
+                                     sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
+      return mp_converged(st, D, R, cte, &ok, &rovf);
+    };
+    FLA_TRY(test());
+    if (!ok) {
+      code = -31;
+      for (it = 1; it <= o.max_iter; ++it) {
+        if (rovf) {
+          code = -2;
+          break;
+        }
+        if (o.refinement == FLA_IR_GMRES) {
+          int gi = 0;
+          FLA_TRY(mp_gmres(st, R, D, V, Wv, o.gmres_restart, o.gmres_tol, &gi));
+          g_mp_stats[5] += gi;
+        } else {
+          FLA_HIP_CHECK(hipMemcpyAsync(D, R, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
+          FLA_TRY(mp_apply_prec(st, D, n));
+        }
+        hipLaunchKernelGGL(axpy_block_kernel, dim3(nblocks(blk)), dim3(256), 0, s, n, nrhs, 1.0,
+                           D, (long)n, X, (long)ldx);
+        FLA_HIP_CHECK(hipGetLastError());
+        FLA_TRY(mp_residual(st, Bd, n, X, ldx, R, n));
+        FLA_TRY(test());
+        if (ok) {
+          code = 0;
+          break;
+        }
+      }
+    }
+  }
+  // spin-wait expiry in a solve: report it rather than a wrong answer
+  {
+    int fl[2] = {0, 0};
+    FLA_HIP_CHECK(hipMemcpyAsync(fl, st.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
+    FLA_HIP_CHECK(hipStreamSynchronize(s));
+    if (fl[1] != 0) return FLA_HIP_ESPIN;
+  }
+  FLA_HIP_CHECK(hipEventRecord(e2, s));
+  if (code == 0) {
+    *iter = it;
+  } else {
+    // FP64 factorization and solve (dsposv.c:403-405): A is overwritten.
+    *iter = code;
+    int dinfo = kSuccess;
+    FLA_TRY(fla_hip_dpotrf(uplo, n, A, lda, &dinfo));
+    if (dinfo != kSuccess) {
+      *info = dinfo + 1;
+    } else {
+      FLA_HIP_CHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, Bd, sizeof(double) * n,
+                                     sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
+      const int t1 = st.upper ? kTrans : kNoTrans, t2 = st.upper ? kNoTrans : kTrans;
+      FLA_TRY(fla_hip_dtrsm(kLeft, uplo, t1, kNonUnit, n, nrhs, 1.0, A, lda, X, ldx));
+      FLA_TRY(fla_hip_dtrsm(kLeft, uplo, t2, kNonUnit, n, nrhs, 1.0, A, lda, X, ldx));
+    }
+  }
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  float ms1 = 0.f, ms2 = 0.f;
+  FLA_HIP_CHECK(hipEventElapsedTime(&ms1, e0, e1));
+  FLA_HIP_CHECK(hipEventElapsedTime(&ms2, e1, e2));
+  g_mp_stats[0] = ms1;
+  g_mp_stats[1] = ms2;
+  g_mp_stats[2] = st.solves;
+  g_mp_stats[3] = st.matvecs;
+  g_mp_stats[4] = anrm;
+  g_mp_stats[6] = finfo;
+  FLA_HIP_CHECK(hipEventDestroy(e0));
+  FLA_HIP_CHECK(hipEventDestroy(e1));
+  FLA_HIP_CHECK(hipEventDestroy(e2));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_mp_last_stats(double* out, int nout) {
+  if (!out || nout < 0) return FLA_HIP_EARG;
+  for (int i = 0; i < nout && i < 8; ++i) out[i] = g_mp_stats[i];
+  return FLA_HIP_OK;
+}

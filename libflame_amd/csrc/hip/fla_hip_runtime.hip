@@ -48,9 +48,11 @@ extern "C" int fla_hip_init(void) {
 extern "C" int fla_hip_finalize(void) {
   if (!g_ctx.ready) return FLA_HIP_OK;
   FLA_HIP_CHECK(hipDeviceSynchronize());
-  if (g_ctx.work) FLA_HIP_CHECK(hipFree(g_ctx.work));
-  g_ctx.work = nullptr;
-  g_ctx.work_bytes = 0;
+  for (int i = 0; i < FlaHipCtx::kSlots; ++i) {
+    if (g_ctx.work[i]) FLA_HIP_CHECK(hipFree(g_ctx.work[i]));
+    g_ctx.work[i] = nullptr;
+    g_ctx.work_bytes[i] = 0;
+  }
   if (g_ctx.dscratch) FLA_HIP_CHECK(hipFree(g_ctx.dscratch));
   g_ctx.dscratch = nullptr;
   if (g_ctx.dlinv) FLA_HIP_CHECK(hipFree(g_ctx.dlinv));
@@ -69,20 +71,24 @@ extern "C" int fla_hip_synchronize(void) {
   return FLA_HIP_OK;
 }
 
-int fla_workspace(size_t bytes, void** ptr) {
+int fla_workspace_slot(int slot, size_t bytes, void** ptr) {
   FLA_TRY(fla_ensure_ready());
-  if (bytes > g_ctx.work_bytes) {
+  if (slot < 0 || slot >= FlaHipCtx::kSlots) return FLA_HIP_EARG;
+  if (bytes > g_ctx.work_bytes[slot]) {
     // Every queued kernel may still use the old buffer.
     FLA_HIP_CHECK(hipDeviceSynchronize());
-    if (g_ctx.work) FLA_HIP_CHECK(hipFree(g_ctx.work));
-    g_ctx.work = nullptr;
+    if (g_ctx.work[slot]) FLA_HIP_CHECK(hipFree(g_ctx.work[slot]));
+    g_ctx.work[slot] = nullptr;
+    g_ctx.work_bytes[slot] = 0;
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
-    FLA_HIP_CHECK(hipMalloc(&g_ctx.work, want));
-    g_ctx.work_bytes = want;
+    FLA_HIP_CHECK(hipMalloc(&g_ctx.work[slot], want));
+    g_ctx.work_bytes[slot] = want;
   }
-  *ptr = g_ctx.work;
+  *ptr = g_ctx.work[slot];
   return FLA_HIP_OK;
 }
+
+int fla_workspace(size_t bytes, void** ptr) { return fla_workspace_slot(0, bytes, ptr); }
 
 extern "C" int fla_hip_malloc(void** ptr, size_t bytes) {
   FLA_TRY(fla_ensure_ready());
@@ -208,9 +214,10 @@ struct StatsState {
   bool on = false;
   std::vector<hipEvent_t> pool;
   std::vector<PendingTiming> pending;
-  double flops[4] = {0, 0, 0, 0};
-  double ms[4] = {0, 0, 0, 0};
-  long launches[4] = {0, 0, 0, 0};
+  static constexpr int kKinds = 8;
+  double flops[kKinds] = {};
+  double ms[kKinds] = {};
+  long launches[kKinds] = {};
 };
 StatsState g_stats;
 
@@ -247,7 +254,8 @@ void fla_stats_harvest() {
   for (auto& p : g_stats.pending) {
     float ms = 0.f;
     if (hipEventSynchronize(p.stop) == hipSuccess &&
-        hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess && p.kind >= 0 && p.kind < 4) {
+        hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess && p.kind >= 0 &&
+        p.kind < StatsState::kKinds) {
       g_stats.ms[p.kind] += ms;
       g_stats.flops[p.kind] += p.flops;
       g_stats.launches[p.kind] += 1;
@@ -265,7 +273,7 @@ extern "C" int fla_hip_stats_enable(int on) {
 
 extern "C" int fla_hip_stats_reset(void) {
   fla_stats_harvest();
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < StatsState::kKinds; ++i) {
     g_stats.flops[i] = 0;
     g_stats.ms[i] = 0;
     g_stats.launches[i] = 0;
@@ -274,7 +282,7 @@ extern "C" int fla_hip_stats_reset(void) {
 }
 
 extern "C" int fla_hip_stats_get(int kind, double* flops, double* ms, long* launches) {
-  if (kind < 0 || kind > 3) return FLA_HIP_EARG;
+  if (kind < 0 || kind >= StatsState::kKinds) return FLA_HIP_EARG;
   fla_stats_harvest();
   if (flops) *flops = g_stats.flops[kind];
   if (ms) *ms = g_stats.ms[kind];

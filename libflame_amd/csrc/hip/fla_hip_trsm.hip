@@ -167,6 +167,16 @@ int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long l
   return FLA_HIP_OK;
 }
 
+int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool upper, bool unit,
+                        double* W) {
+  if (na <= 0) return FLA_HIP_OK;
+  const int nblk = (na + TB - 1) / TB;
+  hipLaunchKernelGGL(trtri_blocks_kernel, dim3(nblk), dim3(256), 0, s, na, A, lda, upper ? 1 : 0,
+                     unit ? 1 : 0, W);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
 static bool is_trans(int t) { return t == kTrans || t == kConjTrans; }
 
 // Scales B by alpha (m x n) when alpha != 1: one MFMA-free elementwise pass.
@@ -203,9 +213,7 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
   void* w = nullptr;
   FLA_TRY(fla_workspace((size_t)nblk * TB * TB * sizeof(double) + 256, &w));
   double* W = static_cast<double*>(w);
-  hipLaunchKernelGGL(trtri_blocks_kernel, dim3(nblk), dim3(256), 0, s, na, A, (long)lda,
-                     upper ? 1 : 0, unit ? 1 : 0, W);
-  FLA_HIP_CHECK(hipGetLastError());
+  FLA_TRY(launch_trtri_blocks(s, na, A, lda, upper, unit, W));
   // op(A) is lower when (lower, N) or (upper, T): forward substitution.
   const bool op_lower = (!upper && !tr) || (upper && tr);
   if (side == kLeft && n <= kNarrowRhs) {

@@ -172,6 +172,61 @@ __global__ __launch_bounds__(256) void matmul_check_kernel(MatSpec S, int n, int
   }
 }
 
+// ---- A = Q diag(s) Q^T with Q a product of Householder reflectors ----------
+
+// A := diag(kappa^(i/(n-1))) (full matrix).
+__global__ void logdiag_kernel(int n, double kappa, double* A, long lda) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  double v = 0.0;
+  if (i == j) v = (n > 1) ? pow(kappa, (double)i / (double)(n - 1)) : 1.0;
+  A[(long)i + (long)j * lda] = v;
+}
+
+// out[0] = v.w, out[1] = v.v (one workgroup, fixed order).
+__global__ void dot2_kernel(int n, const double* __restrict__ v, const double* __restrict__ w,
+                            double* __restrict__ out) {
+  __shared__ double r0[256], r1[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    a += v[i] * w[i];
+    b += v[i] * v[i];
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      r0[threadIdx.x] += r0[threadIdx.x + st];
+      r1[threadIdx.x] += r1[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r0[0];
+    out[1] = r1[0];
+  }
+}
+
+// z = tau w - (tau^2 alpha / 2) v with tau = 2 / (v.v), alpha = v.w.
+__global__ void refl_z_kernel(int n, const double* __restrict__ v, const double* __restrict__ w,
+                              const double* __restrict__ d, double* __restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tau = 2.0 / d[1];
+  z[i] = tau * w[i] - 0.5 * tau * tau * d[0] * v[i];
+}
+
+// A -= v z^T + z v^T
+__global__ void rank2_kernel(int n, const double* __restrict__ v, const double* __restrict__ z,
+                             double* __restrict__ A, long lda) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  A[(long)i + (long)j * lda] -= v[i] * z[j] + z[i] * v[j];
+}
+
 }  // namespace
 
 extern "C" int fla_hip_gen_uniform(unsigned long long seed, int m, int n, double* A, int lda) {
@@ -270,6 +325,63 @@ extern "C" int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, con
   const long total = (long)rows * ncols;
   hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      rows, ncols, perm, tmp, (long)rows, A, (long)lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+// Symmetric positive definite A = Q diag(s) Q^T, s_i = kappa^(i/(n-1)),
+// Q = H_1 ... H_nrefl with H_r = I - 2 v v^T / (v^T v), v ~ N(0,1) drawn by
+// fla_hip_gen_normal(seed + r + 1, n, 1, ...).  Each reflector is applied as
+// A := H A H = A - v z^T - z v^T.  Both triangles are written.
+extern "C" int fla_hip_gen_spd_reflectors(unsigned long long seed, int n, int nrefl, double kappa,
+                                          double* A, int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (n <= 0) return FLA_HIP_OK;
+  if (nrefl < 0 || lda < n || !(kappa >= 1.0)) return FLA_HIP_EARG;
+  hipStream_t s = fla_ctx().s_update;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace_slot(3, sizeof(double) * (3 * (size_t)n + 8), &w));
+  double* v = static_cast<double*>(w);
+  double* wv = v + n;
+  double* z = wv + n;
+  double* d = z + n;
+  const long nn = (long)n * n;
+  hipLaunchKernelGGL(logdiag_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, n,
+                     kappa, A, (long)lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  for (int r = 0; r < nrefl; ++r) {
+    hipLaunchKernelGGL(gen_kernel, dim3((n + 255) / 256), dim3(256), 0, s, seed + r + 1, n, 1, v,
+                       (long)n, 1);
+    FLA_TRY(launch_dgemm_mfma(s, false, false, 0, n, 1, n, 1.0, A, lda, v, n, 0.0, wv, n,
+                              nullptr));
+    hipLaunchKernelGGL(dot2_kernel, dim3(1), dim3(256), 0, s, n, v, wv, d);
+    hipLaunchKernelGGL(refl_z_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, v, wv, d, z);
+    hipLaunchKernelGGL(rank2_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, n, v, z,
+                       A, (long)lda);
+    FLA_HIP_CHECK(hipGetLastError());
+  }
+  return FLA_HIP_OK;
+}
+
+namespace {
+__global__ void d2s_kernel(int m, int n, const double* __restrict__ A, long lda,
+                           float* __restrict__ S, long lds) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  S[(long)i + (long)j * lds] = (float)A[(long)i + (long)j * lda];
+}
+}  // namespace
+
+// S := float(A) for an m x n matrix (round to nearest).  A and S must not
+// overlap.
+extern "C" int fla_hip_convert_d2s(int m, int n, const double* A, int lda, float* S, int lds) {
+  FLA_TRY(fla_ensure_ready());
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || lds < (m > 1 ? m : 1)) return FLA_HIP_EARG;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(d2s_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     fla_ctx().s_update, m, n, A, (long)lda, S, (long)lds);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

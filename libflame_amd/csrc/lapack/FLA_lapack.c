@@ -479,3 +479,88 @@ FLA_Error FLA_LU_piv_solve(FLA_Obj A, FLA_Obj p, FLA_Obj B, FLA_Obj X) {
   if (st != FLA_HIP_OK) return gpu_status(st, "FLA_LU_piv_solve");
   return e == FLA_SUCCESS ? FLA_SUCCESS : e;
 }
+
+/* Reverse-mode adjoint of FLA_Chol (no counterpart in the reference):
+   Abar := (G + G^T)/2, G = L^-T Phi(L^T Lbar) L^-1, for the lower factor L
+   of A = L L^T.  Only FLA_LOWER_TRIANGULAR is supported. */
+FLA_Error FLA_Chol_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj Lbar, FLA_Obj Abar) {
+  fla_devmat dL, dLb, dA;
+  FLA_Error e;
+  int st;
+  if (CHECK_LEVEL_MIN) {
+    FLA_Check_error_code(fla_check_valid_uplo(uplo));
+    FLA_Check_error_code(uplo == FLA_LOWER_TRIANGULAR ? FLA_SUCCESS : FLA_INVALID_UPLO);
+    FLA_Check_error_code(fla_check_floating_object(L));
+    FLA_Check_error_code(fla_check_identical_object_datatype(L, Lbar));
+    FLA_Check_error_code(fla_check_identical_object_datatype(L, Abar));
+    FLA_Check_error_code(fla_check_square(L));
+    FLA_Check_error_code(fla_check_conformal_dims(FLA_NO_TRANSPOSE, L, Lbar));
+    FLA_Check_error_code(fla_check_conformal_dims(FLA_NO_TRANSPOSE, L, Abar));
+  }
+  if (FLA_Obj_has_zero_dim(L)) return FLA_SUCCESS;
+  if ((e = require_double(L)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(L, 1, &dL)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(Lbar, 1, &dLb)) != FLA_SUCCESS) {
+    fla_devmat_release(&dL, 0);
+    return e;
+  }
+  if ((e = fla_devmat_acquire(Abar, 0, &dA)) != FLA_SUCCESS) {
+    fla_devmat_release(&dL, 0);
+    fla_devmat_release(&dLb, 0);
+    return e;
+  }
+  st = fla_hip_dchol_adjoint(dL.m, (const double*)dL.dev, dL.ld, (const double*)dLb.dev, dLb.ld,
+                             (double*)dA.dev, dA.ld);
+  if (st == FLA_HIP_OK) st = fla_hip_synchronize();
+  fla_devmat_release(&dL, 0);
+  fla_devmat_release(&dLb, 0);
+  e = fla_devmat_release(&dA, st == FLA_HIP_OK);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_Chol_adjoint");
+  return e;
+}
+
+/* Reverse-mode adjoint of FLA_Chol_solve (lower): given the solution X of
+   A X = B and Xbar, Bbar := A^-1 Xbar and Abar -= (Bbar X^T + X Bbar^T)/2. */
+FLA_Error FLA_Chol_solve_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj X, FLA_Obj Xbar, FLA_Obj Bbar,
+                                 FLA_Obj Abar) {
+  fla_devmat dL, dX, dXb, dBb, dA;
+  FLA_Error e;
+  int st;
+  if (CHECK_LEVEL_MIN) {
+    FLA_Check_error_code(fla_check_valid_uplo(uplo));
+    FLA_Check_error_code(uplo == FLA_LOWER_TRIANGULAR ? FLA_SUCCESS : FLA_INVALID_UPLO);
+    FLA_Check_error_code(fla_check_floating_object(L));
+    FLA_Check_error_code(fla_check_square(L));
+    FLA_Check_error_code(fla_check_identical_object_datatype(L, X));
+    FLA_Check_error_code(fla_check_conformal_dims(FLA_NO_TRANSPOSE, X, Xbar));
+    FLA_Check_error_code(fla_check_conformal_dims(FLA_NO_TRANSPOSE, X, Bbar));
+    FLA_Check_error_code(fla_check_conformal_dims(FLA_NO_TRANSPOSE, L, Abar));
+  }
+  if (FLA_Obj_has_zero_dim(X)) return FLA_SUCCESS;
+  if ((e = require_double(L)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(L, 1, &dL)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(X, 1, &dX)) != FLA_SUCCESS) goto out1;
+  if ((e = fla_devmat_acquire(Xbar, 1, &dXb)) != FLA_SUCCESS) goto out2;
+  if ((e = fla_devmat_acquire(Bbar, 0, &dBb)) != FLA_SUCCESS) goto out3;
+  if ((e = fla_devmat_acquire(Abar, 1, &dA)) != FLA_SUCCESS) goto out4;
+  st = fla_hip_dchol_solve_adjoint(dL.m, dX.n, (const double*)dL.dev, dL.ld,
+                                   (const double*)dX.dev, dX.ld, (const double*)dXb.dev, dXb.ld,
+                                   (double*)dBb.dev, dBb.ld, (double*)dA.dev, dA.ld);
+  if (st == FLA_HIP_OK) st = fla_hip_synchronize();
+  e = fla_devmat_release(&dA, st == FLA_HIP_OK);
+  fla_devmat_release(&dBb, st == FLA_HIP_OK);
+  fla_devmat_release(&dXb, 0);
+  fla_devmat_release(&dX, 0);
+  fla_devmat_release(&dL, 0);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_Chol_solve_adjoint");
+  return e;
+out4:
+  fla_devmat_release(&dBb, 0);
+out3:
+  fla_devmat_release(&dXb, 0);
+out2:
+  fla_devmat_release(&dX, 0);
+out1:
+  fla_devmat_release(&dL, 0);
+  return e;
+}

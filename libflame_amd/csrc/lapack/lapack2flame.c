@@ -216,3 +216,53 @@ int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, doub
   }
   return 0;
 }
+
+/* dsposv_ (src/map/common/lapacksrc/f2c/dsposv.c:192): reduced-precision
+   Cholesky (FP32 trailing updates) + refinement with an FP64 residual, the
+   netlib stopping rule, 30-step limit and FP64 fall-back (A then holds the
+   FP64 factor).  work / swork are accepted for the ABI and not used: the
+   workspace lives on the device. */
+int dsposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, double* x,
+            int* ldx, double* work, float* swork, int* iter, int* info) {
+  int u = up(*uplo), st;
+  double *dA = NULL, *dB = NULL, *dX = NULL;
+  (void)work;
+  (void)swork;
+  *info = 0;
+  *iter = 0;
+  if (u != 'U' && u != 'L')
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*nrhs < 0)
+    *info = -3;
+  else if (*lda < imax(1, *n))
+    *info = -5;
+  else if (*ldb < imax(1, *n))
+    *info = -7;
+  else if (*ldx < imax(1, *n))
+    *info = -9;
+  if (*info != 0) {
+    xerbla("DSPOSV", -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)*n * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dB, sizeof(double) * (size_t)*n * (size_t)imax(1, *nrhs));
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dX, sizeof(double) * (size_t)*n * (size_t)imax(1, *nrhs));
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *n, a, *lda, dA, *n);
+  if (st == FLA_HIP_OK && *nrhs > 0) st = fla_hip_dmatrix_h2d(*n, *nrhs, b, *ldb, dB, *n);
+  if (st == FLA_HIP_OK)
+    st = fla_hip_dsposv(u == 'U' ? FLA_UPPER_TRIANGULAR : FLA_LOWER_TRIANGULAR, *n, *nrhs, dA, *n,
+                        dB, *n, dX, *n, NULL, iter, info);
+  if (st == FLA_HIP_OK && *nrhs > 0) st = fla_hip_dmatrix_d2h(*n, *nrhs, dX, *n, x, *ldx);
+  if (st == FLA_HIP_OK && *iter < 0) st = fla_hip_dmatrix_d2h(*n, *n, dA, *n, a, *lda);
+  fla_hip_free(dA);
+  fla_hip_free(dB);
+  fla_hip_free(dX);
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: dsposv_ failed: %s\n", fla_hip_error_string(st));
+    *info = -100;
+  }
+  return 0;
+}
