@@ -136,6 +136,9 @@ int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* 
    [3] FP64 products with A, [4] ||A||_inf, [5] GMRES iterations, [6] the
    reduced-precision factorization's info. */
 int fla_hip_mp_last_stats(double* out, int nout);
+/* Panel solve of the reduced-precision factor: 1 (default) multiplies by the
+   inverse of each 128 x 128 diagonal factor, 0 uses blocked substitution. */
+int fla_hip_mp_set_panel(int inverse);
 
 /* ---- batched small factorizations and solves (no counterpart in the
    reference: thousands of independent matrices of uneven sizes in one

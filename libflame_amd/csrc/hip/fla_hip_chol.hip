@@ -384,6 +384,32 @@ int chol_panel_f64(hipStream_t s, int n, double* A, long lda, int k, int kb, int
   return FLA_HIP_OK;
 }
 
+// Same factor with fewer launches, for callers that only need a
+// preconditioner-quality panel (the mixed-precision solver): each 128-block's
+// panel solve multiplies by the explicit inverse of its diagonal factor,
+// A21 := A21 * inv(L11)^T, one in-place MFMA GEMM instead of the blocked
+// substitution (every workgroup of that GEMM owns whole 128-wide rows).
+int chol_panel_f64_inv(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info) {
+  double* W = fla_ctx().dlinv;
+  for (int kk = k; kk < k + kb; kk += PB) {
+    const int ib = (k + kb - kk) < PB ? (k + kb - kk) : PB;
+    double* Akk = A + (long)kk + (long)kk * lda;
+    FLA_TRY(launch_potrf_diag(s, ib, Akk, lda, kk, info));
+    const int mb = n - kk - ib;
+    if (mb <= 0) continue;
+    FLA_TRY(launch_trtri_blocks(s, ib, Akk, lda, false, false, W));
+    FLA_TRY(launch_dgemm_mfma(s, false, true, 0, mb, ib, ib, 1.0, Akk + ib, lda, W, PB, 0.0,
+                              Akk + ib, lda, info));
+    const int w = k + kb - kk - ib;
+    if (w > 0) {
+      const double* P = Akk + ib;
+      FLA_TRY(launch_dgemm_mfma(s, false, true, 1, mb, w, ib, -1.0, P, lda, P, lda, 1.0,
+                                A + (long)(kk + ib) + (long)(kk + ib) * lda, lda, info));
+    }
+  }
+  return FLA_HIP_OK;
+}
+
 // Lower blocked Cholesky with look-ahead: the panel of step k+1 runs on
 // s_panel while the bulk of step k's trailing update runs on s_update.
 static int chol_lower(int n, double* A, long lda, int* dinfo) {

@@ -139,5 +139,8 @@ int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool u
 // FP64 factor of columns [k, k+kb) of a lower matrix (rows k..n-1): diagonal
 // factors, panel solves and in-panel updates, all on stream s.
 int chol_panel_f64(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info);
+// The same with the panel solve done as a product with the explicit inverse
+// of each 128 x 128 diagonal factor (fewer launches, preconditioner quality).
+int chol_panel_f64_inv(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info);
 // In-place exchange of the strictly lower and strictly upper triangles.
 int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

@@ -332,6 +332,11 @@ inline unsigned nblocks(long total) { return (unsigned)((total + 255) / 256); }
 
 // ---- reduced-precision factorization ------------------------------------------
 
+// Panel solve of the reduced-precision factor: product with the inverse of
+// each 128 x 128 diagonal factor (default; fewer launches on the critical
+// path) or blocked substitution (backward stable for any triangle).
+bool g_mp_panel_inverse = true;
+
 struct LpFactor {
   float* As = nullptr;      // FP32 trailing matrix (lower), ld n
   void* P[2] = {};          // packed panels (double-buffered across steps)
@@ -347,7 +352,10 @@ int lp_panel(hipStream_t s, int n, LpFactor& f, double* L, long ldl, int k, int 
                      f.As + (long)k + (long)k * n, (long)n, L + (long)k + (long)k * ldl, ldl,
                      f.amax + slot);
   FLA_HIP_CHECK(hipGetLastError());
-  FLA_TRY(chol_panel_f64(s, n, L, ldl, k, kb, dinfo));
+  if (g_mp_panel_inverse)
+    FLA_TRY(chol_panel_f64_inv(s, n, L, ldl, k, kb, dinfo));
+  else
+    FLA_TRY(chol_panel_f64(s, n, L, ldl, k, kb, dinfo));
   const int m2 = m - kb;
   if (m2 > 0) {
     const double* L21 = L + (long)(k + kb) + (long)k * ldl;
@@ -805,7 +813,7 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
     }
   }
   // spin-wait expiry in a solve: report it rather than a wrong answer
-  {
+  if (st.solves > 0) {
     int fl[2] = {0, 0};
     FLA_HIP_CHECK(hipMemcpyAsync(fl, st.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
     FLA_HIP_CHECK(hipStreamSynchronize(s));
@@ -848,5 +856,10 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
 extern "C" int fla_hip_mp_last_stats(double* out, int nout) {
   if (!out || nout < 0) return FLA_HIP_EARG;
   for (int i = 0; i < nout && i < 8; ++i) out[i] = g_mp_stats[i];
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_mp_set_panel(int inverse) {
+  g_mp_panel_inverse = inverse != 0;
   return FLA_HIP_OK;
 }

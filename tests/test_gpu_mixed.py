@@ -53,9 +53,18 @@ def dev_reflector_spd(seed, n, nrefl, kappa):
     return dA
 
 
+@pytest.fixture
+def substitution_panel():
+    """Exact closed forms need the backward-stable panel solve: the factors of
+    these integer matrices have astronomically large inverses."""
+    fla.lib().fla_hip_mp_set_panel(0)
+    yield
+    fla.lib().fla_hip_mp_set_panel(1)
+
+
 @pytest.mark.parametrize("precision", [fla.FLA_MP_FP32, fla.FLA_MP_BF16])
 @pytest.mark.parametrize("n", [40, 300, 700])
-def test_mp_chol_exact(precision, n):
+def test_mp_chol_exact(precision, n, substitution_panel):
     A, B = exact_spd(n, 5)
     st, info, Lf = mp_chol(A, precision)
     assert st == fla.FLA_HIP_OK and info == fla.FLA_SUCCESS
@@ -63,11 +72,26 @@ def test_mp_chol_exact(precision, n):
 
 
 @pytest.mark.parametrize("precision", [fla.FLA_MP_FP32, fla.FLA_MP_BF16])
-def test_mp_chol_exact_upper(precision):
+def test_mp_chol_exact_upper(precision, substitution_panel):
     A, B = exact_spd(333, 9)
     st, info, Lf = mp_chol(A, precision, uplo=U)
     assert st == fla.FLA_HIP_OK and info == fla.FLA_SUCCESS
     assert np.array_equal(Lf, B)
+
+
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_mp_chol_panel_modes_agree(orc, inverse):
+    """Both panel solves give a factor of FP32 quality on a well-conditioned
+    matrix (the inverse of a diagonal factor is then well conditioned)."""
+    n = 900
+    A = orc.gen_spd_c2(12, n)
+    fla.lib().fla_hip_mp_set_panel(inverse)
+    try:
+        st, info, Lf = mp_chol(A, fla.FLA_MP_FP32)
+    finally:
+        fla.lib().fla_hip_mp_set_panel(1)
+    assert st == fla.FLA_HIP_OK and info == fla.FLA_SUCCESS
+    assert np.linalg.norm(A - Lf @ Lf.T, 1) / np.linalg.norm(A, 1) <= 8 * 2.0 ** -24
 
 
 @pytest.mark.parametrize("precision,bound", [(fla.FLA_MP_FP32, 2.0 ** -24),
@@ -84,7 +108,7 @@ def test_mp_chol_accuracy(orc, precision, bound):
     assert rel >= 2.0 ** -52  # a reduced-precision factor, not an FP64 one
 
 
-def test_mp_chol_failure_index():
+def test_mp_chol_failure_index(substitution_panel):
     """Breakdown is reported at the first non-positive pivot, as FLA_Chol."""
     A, _ = exact_spd(300, 2)
     A[150, 150] = -1.0
