@@ -91,6 +91,9 @@ int fla_hip_set_blocksizes(int chol_nb, int lu_nb);
 int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb);
 /* Tile configuration of the MFMA update kernel (0 = default). */
 int fla_hip_set_gemm_config(int cfg);
+/* Width G (tile columns) of the groups in which update tiles are ordered, so
+   that concurrently running tiles share operand panels in L2 (default 4). */
+int fla_hip_set_gemm_group(int G);
 
 /* ---- measurement ---- */
 /* When enabled, the factorization drivers bracket every trailing-update
@@ -136,8 +139,9 @@ int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* 
    [3] FP64 products with A, [4] ||A||_inf, [5] GMRES iterations, [6] the
    reduced-precision factorization's info. */
 int fla_hip_mp_last_stats(double* out, int nout);
-/* Panel solve of the reduced-precision factor: 1 (default) multiplies by the
-   inverse of each 128 x 128 diagonal factor, 0 uses blocked substitution. */
+/* Panel solve of the reduced-precision factor: 0 (default) blocked
+   substitution, 1 multiplication by the inverse of each 128 x 128 diagonal
+   factor. */
 int fla_hip_mp_set_panel(int inverse);
 
 /* ---- batched small factorizations and solves (no counterpart in the

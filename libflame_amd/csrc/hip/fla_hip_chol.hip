@@ -11,119 +11,12 @@
 // (FLA_Chol_u_blk_var3.c:38-54) is the lower algorithm on A' (the triangles
 // are exchanged in place before and after).
 #include "fla_hip_internal.h"
+#include "fla_hip_lds_tri.h"
 
 namespace {
 
 constexpr int PB = 128;  // panel sub-block width (diagonal block order)
 constexpr int LB = 32;   // leaf width of the panel triangular solve
-
-// Single-workgroup Cholesky of the nb x nb lower block at A (nb <= 128).
-//
-// Every thread owns a 4x4 tile of the lower triangle in registers (528
-// tiles; the padding beyond nb is the identity).  Step k (right-looking, the
-// order of dsyrk/dtrsm-free unblocked Cholesky):
-//   d = sqrt(a_kk)  -- failure when a_kk <= 0 or NaN, as bl1_dsqrte
-//                      (src/base/flamec/include/blis_macro_defs.h:134-136)
-//   l_ik = a_ik * (1/d)  (i > k),  a_ic -= l_ik l_ck  (c > k, i >= c)
-// The diagonal entry of the next step and the scaled column travel through
-// LDS; two barriers per step.  The first failing column is stored in *info
-// (offset by col_offset) and the columns before it are written back.
-__global__ __launch_bounds__(1024) void potrf_tile_kernel(int nb, double* __restrict__ A,
-                                                          long lda, int col_offset,
-                                                          int* __restrict__ info) {
-  if (*info >= 0) return;
-  __shared__ double lcol[PB];
-  __shared__ double dg[2];
-  __shared__ int fail;
-  const int t = threadIdx.x;
-  constexpr int NTILE = (PB / 4) * (PB / 4 + 1) / 2;  // 528
-  const bool own = t < NTILE;
-  int I = 0;
-  if (own) {
-    I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while (I * (I + 1) / 2 > t) --I;
-    while ((I + 1) * (I + 2) / 2 <= t) ++I;
-  }
-  const int J = own ? t - I * (I + 1) / 2 : 0;
-  const int r0 = 4 * I, c0 = 4 * J;
-  double a[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int i = r0 + x, c = c0 + y;
-      double v = 0.0;
-      if (own && i >= c) {
-        if (i < nb && c < nb)
-          v = A[(long)i + (long)c * lda];
-        else if (i == c)
-          v = 1.0;
-      }
-      a[x][y] = v;
-    }
-  if (t == 0) {
-    fail = -1;
-    dg[0] = a[0][0];
-  }
-  __syncthreads();
-  int k = 0;
-  for (; k < PB; ++k) {
-    const int par = k & 1;
-    const double d2 = dg[par];
-    if (!(d2 > 0.0)) {
-      if (t == 0) fail = k;
-      break;
-    }
-    const double d = sqrt(d2);
-    const double inv = 1.0 / d;
-    if (own && c0 <= k && k < c0 + 4) {
-#pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const int i = r0 + x;
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          if (c0 + y != k) continue;
-          if (i > k) {
-            a[x][y] *= inv;
-            lcol[i] = a[x][y];
-          } else if (i == k) {
-            a[x][y] = d;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (own) {
-      const int kn = k + 1;
-#pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const int i = r0 + x;
-        if (i <= k) continue;
-        const double li = lcol[i];
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          const int c = c0 + y;
-          if (c > k && i >= c) {
-            a[x][y] -= li * lcol[c];
-            if (i == kn && c == kn) dg[par ^ 1] = a[x][y];
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-  const int ncols = (fail >= 0) ? fail : nb;
-  if (own) {
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        const int i = r0 + x, c = c0 + y;
-        if (i < nb && c < ncols && i >= c) A[(long)i + (long)c * lda] = a[x][y];
-      }
-  }
-  if (t == 0 && fail >= 0) *info = col_offset + fail;
-}
 
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -144,23 +37,32 @@ constexpr int DLD = PB + 1;  // LDS leading dimension of the diagonal block
 //      32x32 factor read from LDS by broadcast), dtrsm's update order;
 //   3. the trailing part is updated with 4x4 register tiles (rank 32).
 // The block lives in LDS (129 KB); padding beyond nb is the identity.
+// INV: after a successful factor, also writes inv(L) (lower, ld 128, zero
+// upper part) to W, inverted in LDS (fla_hip_lds_tri.h).
+template <bool INV>
 __global__ __launch_bounds__(256) void potrf_blk_kernel(int nb, double* __restrict__ A, long lda,
-                                                        int col_offset, int* __restrict__ info) {
+                                                        int col_offset, int* __restrict__ info,
+                                                        double* __restrict__ W) {
   if (*info >= 0) return;
   __shared__ double S[PB * DLD];
   __shared__ int fail;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  for (int e = t; e < PB * PB; e += 256) {
-    const int i = e & (PB - 1), c = e >> 7;
-    double v = 0.0;
-    if (i >= c) {
-      if (i < nb && c < nb)
-        v = A[(long)i + (long)c * lda];
-      else if (i == c)
-        v = 1.0;
+  // 64 elements per thread, 16 loads in flight at a time
+  for (int q0 = 0; q0 < PB * PB / 256; q0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = t + (q0 + q) * 256;
+      const int i = e & (PB - 1), c = e >> 7;
+      v[q] = (i < nb && c < nb) ? A[(long)i + (long)c * lda] : 0.0;
     }
-    S[c * DLD + i] = v;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = t + (q0 + q) * 256;
+      const int i = e & (PB - 1), c = e >> 7;
+      S[c * DLD + i] = (i < c) ? 0.0 : ((i < nb) ? v[q] : (i == c ? 1.0 : 0.0));
+    }
   }
   if (t == 0) fail = -1;
   __syncthreads();
@@ -254,6 +156,15 @@ __global__ __launch_bounds__(256) void potrf_blk_kernel(int nb, double* __restri
     if (i >= c && i < nb && c < ncols) A[(long)i + (long)c * lda] = S[c * DLD + i];
   }
   if (t == 0 && fail >= 0) *info = col_offset + fail;
+  if (INV && fail < 0) {
+    // the strictly upper part of S is zero and the padding the identity
+    __syncthreads();
+    fla_lds::trtri_lower_lds(S, t);
+    for (int e = t; e < PB * PB; e += 256) {
+      const int i = e & (PB - 1), c = e >> 7;
+      W[i + c * PB] = (i >= c) ? S[c * DLD + i] : 0.0;
+    }
+  }
 }
 
 // Leaf of the panel solve: X := X * inv(L)' for an m x nb strip (nb <= 32),
@@ -317,21 +228,87 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 }  // namespace
 
+bool g_trsm_rows = true;
+
 int launch_potrf_diag(hipStream_t s, int nb, double* A, long lda, int col_offset, int* info) {
   if (nb <= 0) return FLA_HIP_OK;
   if (nb > PB) return FLA_HIP_EARG;
-  hipLaunchKernelGGL(potrf_blk_kernel, dim3(1), dim3(256), 0, s, nb, A, lda, col_offset, info);
+  hipLaunchKernelGGL(potrf_blk_kernel<false>, dim3(1), dim3(256), 0, s, nb, A, lda, col_offset,
+                     info, nullptr);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+// The same, and W (128 x 128, ld 128) := inv(L) when the factor succeeds.
+int launch_potrf_diag_inv(hipStream_t s, int nb, double* A, long lda, int col_offset, int* info,
+                          double* W) {
+  if (nb <= 0) return FLA_HIP_OK;
+  if (nb > PB) return FLA_HIP_EARG;
+  hipLaunchKernelGGL(potrf_blk_kernel<true>, dim3(1), dim3(256), 0, s, nb, A, lda, col_offset,
+                     info, W);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
 
 // A21 := A21 * inv(L11)' for an m x nb strip, L11 lower non-unit (nb <= 128):
-// blocked substitution -- leaves of 32 columns solved in registers, each
-// followed by an MFMA update of the strip's remaining columns.
+// one launch, one thread per row of the strip.  Columns are solved in blocks
+// of 32 held in VGPRs: first the contributions of the already solved columns
+// k (ascending, re-read by the thread that wrote them), then the 32 x 32
+// diagonal block by substitution -- dtrsm_('R','L','T','N')'s update order
+// (src/flablas/f2c/dtrsm.c), temp = 1/a_kk multiplied in.  L is read with
+// wave-uniform (scalar) loads.
+template <bool FULL>
+__global__ __launch_bounds__(256) void trsm_rlt_rows_kernel(int m, int nb,
+                                                            const double* __restrict__ L,
+                                                            long ldl, double* __restrict__ X,
+                                                            long ldx, const int* __restrict__ abort) {
+  if (abort && *abort >= 0) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= m) return;
+  double* xr = X + row;
+  const int nbk = FULL ? PB : nb;
+  for (int jb = 0; jb < nbk; jb += LB) {
+    const int w = FULL ? LB : ((nbk - jb) < LB ? (nbk - jb) : LB);
+    double x[LB];
+#pragma unroll
+    for (int c = 0; c < LB; ++c) x[c] = (FULL || c < w) ? xr[(long)(jb + c) * ldx] : 0.0;
+    for (int k = 0; k < jb; ++k) {
+      const double xk = xr[(long)k * ldx];
+      const double* lk = L + (long)k * ldl + jb;
+#pragma unroll
+      for (int c = 0; c < LB; ++c)
+        if (FULL || c < w) x[c] -= lk[c] * xk;
+    }
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      if (FULL || j < w) {
+        x[j] *= 1.0 / L[(long)(jb + j) + (long)(jb + j) * ldl];
+#pragma unroll
+        for (int c = j + 1; c < LB; ++c)
+          if (FULL || c < w) x[c] -= L[(long)(jb + c) + (long)(jb + j) * ldl] * x[j];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < LB; ++c)
+      if (FULL || c < w) xr[(long)(jb + c) * ldx] = x[c];
+  }
+}
+
 int launch_trsm_rlt(hipStream_t s, int m, int nb, const double* L11, long ldl, double* A21,
                     long lda, const int* abort_flag) {
   if (m <= 0 || nb <= 0) return FLA_HIP_OK;
   if (nb > PB) return FLA_HIP_EARG;
+  if (g_trsm_rows) {
+    const unsigned g = (unsigned)((m + 255) / 256);
+    if (nb == PB)
+      hipLaunchKernelGGL(trsm_rlt_rows_kernel<true>, dim3(g), dim3(256), 0, s, m, nb, L11, ldl,
+                         A21, lda, abort_flag);
+    else
+      hipLaunchKernelGGL(trsm_rlt_rows_kernel<false>, dim3(g), dim3(256), 0, s, m, nb, L11, ldl,
+                         A21, lda, abort_flag);
+    FLA_HIP_CHECK(hipGetLastError());
+    return FLA_HIP_OK;
+  }
   for (int b0 = 0; b0 < nb; b0 += LB) {
     const int w = (nb - b0) < LB ? (nb - b0) : LB;
     const double* Lbb = L11 + (long)b0 + (long)b0 * ldl;
@@ -394,10 +371,12 @@ int chol_panel_f64_inv(hipStream_t s, int n, double* A, long lda, int k, int kb,
   for (int kk = k; kk < k + kb; kk += PB) {
     const int ib = (k + kb - kk) < PB ? (k + kb - kk) : PB;
     double* Akk = A + (long)kk + (long)kk * lda;
-    FLA_TRY(launch_potrf_diag(s, ib, Akk, lda, kk, info));
     const int mb = n - kk - ib;
-    if (mb <= 0) continue;
-    FLA_TRY(launch_trtri_blocks(s, ib, Akk, lda, false, false, W));
+    if (mb <= 0) {
+      FLA_TRY(launch_potrf_diag(s, ib, Akk, lda, kk, info));
+      continue;
+    }
+    FLA_TRY(launch_potrf_diag_inv(s, ib, Akk, lda, kk, info, W));
     FLA_TRY(launch_dgemm_mfma(s, false, true, 0, mb, ib, ib, 1.0, Akk + ib, lda, W, PB, 0.0,
                               Akk + ib, lda, info));
     const int w = k + kb - kk - ib;
@@ -483,5 +462,10 @@ extern "C" int fla_hip_dpotrf(int uplo, int n, double* A, int lda, int* info) {
   FLA_HIP_CHECK(hipStreamSynchronize(c.s_update));
   fla_stats_harvest();
   *info = h;
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_set_panel_solve(int rows) {
+  g_trsm_rows = rows != 0;
   return FLA_HIP_OK;
 }

@@ -156,7 +156,7 @@ int launch_cfg(hipStream_t s, bool ta, bool tb, int mode, int m, int n, int k, d
                long ldc, const int* abort_flag) {
   static_assert(BM == BN, "the lower-tile enumeration assumes square tiles");
   const int Tm = (m + BM - 1) / BM, Tn = (n + BN - 1) / BN;
-  const int rpc_eff = 1;  // column tile c starts at row tile c
+  const int rpc_eff = g_tile_group;  // tile-column group width (fla_tile_of)
   const long nblk = (mode == 0) ? (long)Tm * Tn : (long)Tn * Tm - (long)Tn * (Tn - 1) / 2;
   if (nblk <= 0) return FLA_HIP_OK;
   if (nblk > 0x7fffffffL) return FLA_HIP_EARG;
@@ -179,6 +179,12 @@ int launch_cfg(hipStream_t s, bool ta, bool tb, int mode, int m, int n, int k, d
 }
 
 }  // namespace
+
+extern "C" int fla_hip_set_gemm_group(int G) {
+  if (G < 1 || G > 64) return FLA_HIP_EARG;
+  g_tile_group = G;
+  return FLA_HIP_OK;
+}
 
 extern "C" int fla_hip_set_gemm_config(int cfg) {
   if (cfg < 0 || cfg > 7) return FLA_HIP_EARG;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cmath>
 #include <cstdio>
 
 #include "FLA_hip.h"
@@ -82,33 +83,66 @@ inline double lower_update_flops(int m, int n, int k) {
 
 // Maps a linear workgroup id to a C tile.  Workgroups are dealt to the eight
 // XCDs round-robin; the id is first permuted so that each XCD receives a
-// contiguous run of tiles (which share operand panels in its own L2).
-// mode 0: column-major over a Tm x Tn grid; mode 1: lower tiles (whose rows
-// reach the diagonal) of a Tm x Tn grid, column by column.
-__device__ __forceinline__ void fla_tile_of(int bid, int nblk, int mode, int Tm, int Tn, int rpc,
-                                        int& ti, int& tj) {
-  const int x = bid & 7;
-  const int q = nblk >> 3, rr = nblk & 7;
-  const int t = x * q + (x < rr ? x : rr) + (bid >> 3);
+// contiguous run of tiles.  Tiles are ordered by groups of G tile columns
+// (row by row inside a group), so the workgroups an XCD runs at the same time
+// share both operand panels in its L2: a G x (32/G) patch of C reads G
+// column panels and 32/G row panels instead of 1 and 32.
+// mode 0: all Tm x Tn tiles; mode 1: the lower tiles (ti >= tj) of a
+// Tm x Tn grid whose diagonal starts at the top-left (Tm >= Tn).
+__host__ __device__ inline long fla_group_offset(int s, int G, int Tm, int mode) {
+  // tiles before group s (all of them G columns wide)
+  if (mode == 0) return (long)s * G * Tm;
+  return (long)s * G * Tm - (long)G * G * s * (s - 1) / 2 - (long)s * G * (G - 1) / 2;
+}
+
+__host__ __device__ inline void fla_tile_of_linear(long t, int mode, int Tm, int Tn, int G,
+                                                   int& ti, int& tj) {
+  // group s: largest s with offset(s) <= t
+  int s;
   if (mode == 0) {
-    tj = t / Tm;
-    ti = t - tj * Tm;
+    s = (int)(t / ((long)G * Tm));
   } else {
-    // Column c holds the row tiles [c*rpc, Tm): offset(c) = c*Tm - rpc*c*(c-1)/2.
-    int c = 0;
-    {
-      const double a = 2.0 * Tm + rpc;
-      const double disc = a * a - 8.0 * rpc * (double)t;
-      c = (int)((a - sqrt(disc > 0.0 ? disc : 0.0)) / (2.0 * rpc));
-    }
-    if (c < 0) c = 0;
-    if (c > Tn - 1) c = Tn - 1;
-    while (c > 0 && (long)c * Tm - (long)rpc * c * (c - 1) / 2 > t) --c;
-    while (c + 1 < Tn && (long)(c + 1) * Tm - (long)rpc * (c + 1) * c / 2 <= t) ++c;
-    tj = c;
-    ti = c * rpc + (t - (c * Tm - rpc * c * (c - 1) / 2));
+    const double a = 0.5 * G * G, bq = (double)G * Tm + 0.5 * G;
+    const double disc = bq * bq - 4.0 * a * (double)t;
+    s = (int)((bq - sqrt(disc > 0.0 ? disc : 0.0)) / (2.0 * a));
+  }
+  const int ng = (Tn + G - 1) / G;
+  if (s < 0) s = 0;
+  if (s > ng - 1) s = ng - 1;
+  while (s > 0 && fla_group_offset(s, G, Tm, mode) > t) --s;
+  while (s + 1 < ng && fla_group_offset(s + 1, G, Tm, mode) <= t) ++s;
+  const long u = t - fla_group_offset(s, G, Tm, mode);
+  const int c0 = s * G;
+  const int w = (Tn - c0) < G ? (Tn - c0) : G;
+  if (mode == 0) {
+    ti = (int)(u / w);
+    tj = c0 + (int)(u % w);
+    return;
+  }
+  const long tri = (long)w * (w + 1) / 2;
+  if (u < tri) {
+    int r = (int)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while ((long)r * (r + 1) / 2 > u) --r;
+    while ((long)(r + 1) * (r + 2) / 2 <= u) ++r;
+    ti = c0 + r;
+    tj = c0 + (int)(u - (long)r * (r + 1) / 2);
+  } else {
+    const long v = u - tri;
+    ti = c0 + w + (int)(v / w);
+    tj = c0 + (int)(v % w);
   }
 }
+
+__device__ __forceinline__ void fla_tile_of(int bid, int nblk, int mode, int Tm, int Tn, int G,
+                                            int& ti, int& tj) {
+  const int x = bid & 7;
+  const int q = nblk >> 3, rr = nblk & 7;
+  const long t = (long)x * q + (x < rr ? x : rr) + (bid >> 3);
+  fla_tile_of_linear(t, mode, Tm, Tn, G, ti, tj);
+}
+
+// Tile-column group width of the update kernels' tile order (fla_tile_of).
+extern int g_tile_group;
 
 // ---- kernel launchers implemented in the other translation units ----------
 // MFMA FP64 update: C := alpha*op(A)*op(B) + beta*C (mode 0 full, 1 lower).

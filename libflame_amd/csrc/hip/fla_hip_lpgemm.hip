@@ -102,10 +102,11 @@ __global__ __launch_bounds__(256) void lp_update_kernel(int m, int n, int nkb,
                                                         float alpha,
                                                         const unsigned* __restrict__ amax_bits,
                                                         float* __restrict__ C, long ldc, int Tm,
-                                                        int Tn, const int* __restrict__ abort) {
+                                                        int Tn, int group,
+                                                        const int* __restrict__ abort) {
   if (abort && *abort != kSuccess) return;
   int ti, tj;
-  fla_tile_of(blockIdx.x, gridDim.x, LOWER ? 1 : 0, Tm, Tn, 1, ti, tj);
+  fla_tile_of(blockIdx.x, gridDim.x, LOWER ? 1 : 0, Tm, Tn, group, ti, tj);
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int wr = w >> 1, wc = w & 1;
   const int i0 = ti * 128 + wr * 64, j0 = tj * 128 + wc * 64;
@@ -244,16 +245,16 @@ int launch_lp_update(hipStream_t s, bool bf16, bool lower, int m, int n, int k, 
   const dim3 grid((unsigned)nblk), block(256);
   if (bf16 && lower)
     hipLaunchKernelGGL((lp_update_kernel<true, true>), grid, block, 0, s, m, n, nkb, P, prow0 / 16,
-                       Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, abort_flag);
+                       Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   else if (bf16)
     hipLaunchKernelGGL((lp_update_kernel<true, false>), grid, block, 0, s, m, n, nkb, P,
-                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, abort_flag);
+                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   else if (lower)
     hipLaunchKernelGGL((lp_update_kernel<false, true>), grid, block, 0, s, m, n, nkb, P,
-                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, abort_flag);
+                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   else
     hipLaunchKernelGGL((lp_update_kernel<false, false>), grid, block, 0, s, m, n, nkb, P,
-                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, abort_flag);
+                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

@@ -219,17 +219,25 @@ __global__ void syminf_partial_kernel(int n, const double* __restrict__ A, long 
   }
 }
 
-// out[0] = max_i (sum_{bj <= bi(i)} Prow[bj][i] + sum_{bi >= bj(i)} Pcol[bi][i]).
-__global__ void syminf_finish_kernel(int n, const double* __restrict__ Prow,
-                                     const double* __restrict__ Pcol, double* out) {
+// Prow[i] := sum_{bj <= bi(i)} Prow[bj][i] + sum_{bi >= bj(i)} Pcol[bi][i]
+// (row i of |A| summed; thread i reads and then overwrites only column i).
+__global__ void syminf_rows_kernel(int n, double* __restrict__ Prow,
+                                   const double* __restrict__ Pcol) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int T = (n + 63) / 64, b = i / 64;
+  double s = 0.0;
+  for (int bj = 0; bj <= b; ++bj) s += Prow[(long)bj * n + i];
+  for (int bi = b; bi < T; ++bi) s += Pcol[(long)bi * n + i];
+  Prow[i] = s;
+}
+
+// out[0] = max_i v[i] (NaN propagates; one workgroup).
+__global__ void vmax_kernel(int n, const double* __restrict__ v, double* out) {
   __shared__ double red[256];
-  const int T = (n + 63) / 64;
   double mx = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) {
-    const int b = i / 64;
-    double s = 0.0;
-    for (int bj = 0; bj <= b; ++bj) s += Prow[(long)bj * n + i];
-    for (int bi = b; bi < T; ++bi) s += Pcol[(long)bi * n + i];
+    const double s = v[i];
     mx = (s > mx || s != s) ? s : mx;
   }
   red[threadIdx.x] = mx;
@@ -332,10 +340,10 @@ inline unsigned nblocks(long total) { return (unsigned)((total + 255) / 256); }
 
 // ---- reduced-precision factorization ------------------------------------------
 
-// Panel solve of the reduced-precision factor: product with the inverse of
-// each 128 x 128 diagonal factor (default; fewer launches on the critical
-// path) or blocked substitution (backward stable for any triangle).
-bool g_mp_panel_inverse = true;
+// Panel solve of the reduced-precision factor: blocked substitution
+// (default; backward stable for any triangle) or the product with the
+// inverse of each 128 x 128 diagonal factor.
+bool g_mp_panel_inverse = false;
 
 struct LpFactor {
   float* As = nullptr;      // FP32 trailing matrix (lower), ld n
@@ -744,7 +752,8 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   double* Pcol = Prow + (long)T64 * n;
   hipLaunchKernelGGL(syminf_partial_kernel, dim3((unsigned)((long)T64 * (T64 + 1) / 2)),
                      dim3(256), 0, s, n, A, (long)lda, st.upper, Prow, Pcol);
-  hipLaunchKernelGGL(syminf_finish_kernel, dim3(1), dim3(256), 0, s, n, Prow, Pcol, st.hbuf);
+  hipLaunchKernelGGL(syminf_rows_kernel, dim3(nblocks(n)), dim3(256), 0, s, n, Prow, Pcol);
+  hipLaunchKernelGGL(vmax_kernel, dim3(1), dim3(256), 0, s, n, Prow, st.hbuf);
   FLA_HIP_CHECK(hipGetLastError());
   double anrm = 0.0;
   FLA_HIP_CHECK(hipMemcpyAsync(&anrm, st.hbuf, sizeof(double), hipMemcpyDeviceToHost, s));

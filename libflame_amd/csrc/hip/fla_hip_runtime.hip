@@ -11,6 +11,8 @@
 
 static FlaHipCtx g_ctx;
 
+int g_tile_group = 4;
+
 FlaHipCtx& fla_ctx() { return g_ctx; }
 
 int fla_ensure_ready() {

@@ -14,52 +14,12 @@
 // dtrsm_('L','L','N','U')) runs as one lane per column with the 128-entry
 // column in VGPRs and L11 read through uniform loads.
 #include "fla_hip_internal.h"
+#include "fla_hip_lds_tri.h"
 
 namespace {
 
 constexpr int TB = 128;       // diagonal block order
 constexpr int TLD = TB + 1;   // LDS leading dimension
-
-// One level of the recursive inversion in LDS: for every 2H-block at offset o,
-// the lower-left H x H block B becomes -inv(C) * B * inv(A) (A, C already
-// inverted).  Each of the 256 threads holds H/4 outputs in registers.
-template <int H>
-__device__ __forceinline__ void trtri_combine(double* S, int t) {
-  constexpr int PER = H / 4;
-  double reg[PER];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int e = t + 256 * q;
-    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
-    double sum = 0.0;
-    for (int kk = j; kk < H; ++kk) sum += S[(o + kk) * TLD + o + H + i] * S[(o + j) * TLD + o + kk];
-    reg[q] = sum;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int e = t + 256 * q;
-    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
-    S[(o + j) * TLD + o + H + i] = reg[q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int e = t + 256 * q;
-    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
-    double sum = 0.0;
-    for (int kk = 0; kk <= i; ++kk) sum += S[(o + H + kk) * TLD + o + H + i] * S[(o + j) * TLD + o + H + kk];
-    reg[q] = -sum;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int e = t + 256 * q;
-    const int i = e % H, j = (e / H) % H, o = (e / (H * H)) * 2 * H;
-    S[(o + j) * TLD + o + H + i] = reg[q];
-  }
-  __syncthreads();
-}
 
 // Inverts the nbk x nbk diagonal block k of a triangular matrix (lower or
 // upper storage, unit or non-unit diagonal) into W (ld TB per block, same
@@ -78,41 +38,28 @@ __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* 
   const int r0 = k * TB;
   const int nbk = (n - r0) < TB ? (n - r0) : TB;
   const int t = threadIdx.x;
-  for (int idx = t; idx < TB * TB; idx += 256) {
-    const int i = idx % TB, j = idx / TB;  // S(i,j) lower
-    double v = 0.0;
-    if (i < nbk && j < nbk && i >= j) {
-      v = upper ? A[(long)(r0 + j) + (long)(r0 + i) * lda] : A[(long)(r0 + i) + (long)(r0 + j) * lda];
-      if (i == j && unit) v = 1.0;
+  // 64 elements per thread, loads issued 16 at a time; consecutive threads
+  // read consecutive addresses of the stored triangle.
+  for (int q0 = 0; q0 < TB * TB / 256; q0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = t + (q0 + q) * 256;
+      const int f = idx % TB, g = idx / TB;  // stored element (f, g)
+      v[q] = (f < nbk && g < nbk) ? A[(long)(r0 + f) + (long)(r0 + g) * lda] : 0.0;
     }
-    if (i >= nbk && i == j) v = 1.0;  // identity padding
-    S[j * TLD + i] = v;
-  }
-  __syncthreads();
-  {
-    const int w = t >> 6, lane = t & 63;
-    if (lane < 32) {
-      const int o = 32 * w, c = lane;
-      double y[32];
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        double sum = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-        for (int kk = 0; kk < i; ++kk) sum -= S[(o + kk) * TLD + o + i] * y[kk];
-        y[i] = sum / S[(o + i) * TLD + o + i];
-        // Pins y[i] (and the LDS reads) to this iteration: otherwise the
-        // compiler hoists every broadcast read of later rows and spills.
-        asm volatile("" : "+v"(y[i])::"memory");
-      }
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int i = 0; i < 32; ++i)
-        if (i >= c) S[(o + c) * TLD + o + i] = y[i];
+    for (int q = 0; q < 16; ++q) {
+      const int idx = t + (q0 + q) * 256;
+      const int f = idx % TB, g = idx / TB;
+      const int i = upper ? g : f, j = upper ? f : g;  // S(i, j), lower
+      double x = (i >= j && i < nbk) ? v[q] : 0.0;
+      if (i == j && (unit || i >= nbk)) x = 1.0;  // unit diagonal / identity padding
+      S[j * TLD + i] = x;
     }
   }
   __syncthreads();
-  trtri_combine<32>(S, t);
-  trtri_combine<64>(S, t);
+  fla_lds::trtri_lower_lds(S, t);
   double* Wk = W + (long)k * TB * TB;
   for (int idx = t; idx < TB * TB; idx += 256) {
     const int i = idx % TB, j = idx / TB;  // W(i,j), ld TB
