@@ -47,6 +47,19 @@ def make_lu(fla, n, seed):
     return A
 
 
+def profiled_traffic():
+    """HBM bytes per trailing-update launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "chol_n32768_update_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy(fla, dst, src):
     fla.check(fla.lib().fla_hip_memcpy_d2d(dst.ptr, src.ptr, src.ld * src.n * 8), "copy")
 
@@ -71,12 +84,16 @@ def chol_probe_residual(fla, np, A0, L, n, nprobe=4):
                      for j in range(nprobe)))
 
 
-def cpu_baseline(n_cpu):
+def cpu_baseline(fla, n_cpu):
     """The oracle's FLA_Chol (libflame's control tree, f2c loop order, one
-    thread) on a bounded n_cpu sample of the same matrix family."""
+    thread) on a bounded n_cpu sample of the same matrix family, generated on
+    the GPU (A = B*B'/n + I) and copied to the host."""
     import numpy as np
     from oracle import oracle as orc
-    A = orc.gen_spd_c2(11, n_cpu)
+    dA = make_spd(fla, n_cpu, 11)
+    A = dA.download()
+    A = np.asfortranarray(np.tril(A) + np.tril(A, -1).T)
+    del dA
     t0 = time.perf_counter()
     _, r = orc.chol(A)
     dt = time.perf_counter() - t0
@@ -301,7 +318,7 @@ def main():
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--chol-nb", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=3072)
+    ap.add_argument("--cpu-n", type=int, default=5120)
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -413,7 +430,7 @@ def main():
     if not args.no_adjoint:
         extras["adjoint"] = bench_adjoint(fla, np, args.adj_n)
 
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
+    cpu = None if args.no_cpu else cpu_baseline(fla, args.cpu_n)
     out = {
         "metric": "Chol/LU TFLOP/s @ n=32768 FP64 vs mixed-prec+IR time-to-solve at equal bwd err",
         "value": round(tflops, 3),
@@ -431,7 +448,8 @@ def main():
                    "n": n, "blocksize": fla.get_blocksizes()[0], "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
-                     "traffic": None, "kernel": "dgemm_mfma_kernel<false,true> (lower SYRK update)",
+                     "traffic": profiled_traffic(),
+                     "kernel": "dgemm_mfma_kernel<false,true> (lower SYRK update)",
                      "launches": upd_n},
         "cpu_baseline": cpu,
         "extras": extras,

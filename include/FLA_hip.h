@@ -94,6 +94,9 @@ int fla_hip_set_gemm_config(int cfg);
 /* Width G (tile columns) of the groups in which update tiles are ordered, so
    that concurrently running tiles share operand panels in L2 (default 4). */
 int fla_hip_set_gemm_group(int G);
+/* CUs left out of the update stream's CU mask for the panel stream
+   (default 8; 0 = no mask). */
+int fla_hip_set_reserved_cus(int reserved);
 
 /* ---- measurement ---- */
 /* When enabled, the factorization drivers bracket every trailing-update

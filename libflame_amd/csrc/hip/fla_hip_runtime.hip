@@ -15,6 +15,32 @@ int g_tile_group = 4;
 
 FlaHipCtx& fla_ctx() { return g_ctx; }
 
+// The update stream may leave the last `reserved` CUs to the panel stream:
+// the trailing update launches thousands of LDS-heavy workgroups, and a
+// single-workgroup panel kernel that needs a whole CU's LDS would otherwise
+// wait for a CU to drain completely.
+static int g_reserved_cus = 8;
+
+static int create_update_stream(int reserved) {
+  if (reserved <= 0 || reserved >= g_ctx.num_cus) {
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+    return FLA_HIP_OK;
+  }
+  const int words = (g_ctx.num_cus + 31) / 32;
+  uint32_t mask[16] = {};
+  if (words > 16) {
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+    return FLA_HIP_OK;
+  }
+  const int use = g_ctx.num_cus - reserved;
+  for (int c = 0; c < use; ++c) mask[c / 32] |= 1u << (c % 32);
+  if (hipExtStreamCreateWithCUMask(&g_ctx.s_update, (uint32_t)words, mask) != hipSuccess) {
+    (void)hipGetLastError();
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+  }
+  return FLA_HIP_OK;
+}
+
 int fla_ensure_ready() {
   if (g_ctx.ready) return FLA_HIP_OK;
   return fla_hip_init();
@@ -37,7 +63,7 @@ extern "C" int fla_hip_init(void) {
   }
   g_ctx.num_cus = prop.multiProcessorCount;
   // Default-priority streams only.
-  FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+  FLA_TRY(create_update_stream(g_reserved_cus));
   FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_panel, hipStreamNonBlocking));
   for (int i = 0; i < FlaHipCtx::kEvents; ++i)
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
@@ -290,4 +316,14 @@ extern "C" int fla_hip_stats_get(int kind, double* flops, double* ms, long* laun
   if (ms) *ms = g_stats.ms[kind];
   if (launches) *launches = g_stats.launches[kind];
   return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_set_reserved_cus(int reserved) {
+  if (reserved < 0) return FLA_HIP_EARG;
+  g_reserved_cus = reserved;
+  if (!g_ctx.ready) return FLA_HIP_OK;
+  FLA_HIP_CHECK(hipDeviceSynchronize());
+  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
+  g_ctx.s_update = nullptr;
+  return create_update_stream(reserved);
 }
