@@ -445,7 +445,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "FLA_Chol lower FP64, n=32768, A=B*B'/n+I, B~N(0,1)",
-                   "n": n, "blocksize": fla.get_blocksizes()[0], "parallelism": "single-gpu"},
+                   "n": n, "blocksize": fla.lib().fla_hip_chol_blocksize(n),
+                   "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
                      "traffic": profiled_traffic(),

@@ -89,6 +89,9 @@ int fla_hip_shift_pivots(int ptype, int m, int* p);
 /* Outer (trailing-update) and inner (panel) block sizes; 0 keeps current. */
 int fla_hip_set_blocksizes(int chol_nb, int lu_nb);
 int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb);
+/* Outer Cholesky block used for order n (chol_nb 0 = automatic: n/32
+   rounded to a multiple of 128 within [256, 1024]). */
+int fla_hip_chol_blocksize(int n);
 /* Tile configuration of the MFMA update kernel (0 = default). */
 int fla_hip_set_gemm_config(int cfg);
 /* Width G (tile columns) of the groups in which update tiles are ordered, so

@@ -108,6 +108,7 @@ def lib():
             "fla_hip_stats_get": (C.c_int, [C.c_int, dp, dp, C.POINTER(C.c_long)]),
             "fla_hip_set_blocksizes": (C.c_int, [C.c_int, C.c_int]),
             "fla_hip_get_blocksizes": (C.c_int, [ip, ip]),
+            "fla_hip_chol_blocksize": (C.c_int, [C.c_int]),
             "fla_hip_gen_uniform": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, vp, C.c_int]),
             "fla_hip_gen_normal": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, vp, C.c_int]),
             "fla_hip_gram_lower": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_double,

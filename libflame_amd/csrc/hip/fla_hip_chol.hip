@@ -393,7 +393,7 @@ int chol_panel_f64_inv(hipStream_t s, int n, double* A, long lda, int k, int kb,
 // s_panel while the bulk of step k's trailing update runs on s_update.
 static int chol_lower(int n, double* A, long lda, int* dinfo) {
   FlaHipCtx& c = fla_ctx();
-  const int nb = c.chol_nb;
+  const int nb = fla_chol_nb(n);
   hipStream_t sp = c.s_panel, su = c.s_update;
   int evi = 0;
   auto next_event = [&]() { hipEvent_t e = c.ev[evi]; evi = (evi + 1) % FlaHipCtx::kEvents; return e; };

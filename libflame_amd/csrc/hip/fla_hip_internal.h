@@ -43,11 +43,15 @@ struct FlaHipCtx {
   int* dscratch = nullptr;
   // 128 x 128 FP64 block: inverse of the current Cholesky diagonal block.
   double* dlinv = nullptr;
-  int chol_nb = 256;
+  int chol_nb = 0;  // 0: chosen from n (fla_chol_nb)
   int lu_nb = 128;
 };
 
 FlaHipCtx& fla_ctx();
+// Outer Cholesky block for order n: the configured one, else n/32 rounded
+// down to a multiple of 128 and clamped to [256, 1024] (measured on MI355X:
+// larger blocks raise the trailing update's arithmetic intensity).
+int fla_chol_nb(int n);
 int fla_ensure_ready();
 // Returns a device workspace of at least `bytes` bytes (grow-only).
 int fla_workspace(size_t bytes, void** ptr);

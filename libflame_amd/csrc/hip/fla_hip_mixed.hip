@@ -431,7 +431,7 @@ int mp_chol(int upper, int n, const double* A, long lda, bool bf16, double* L, l
             int* info, int* ovf) {
   FlaHipCtx& c = fla_ctx();
   hipStream_t s = c.s_update;
-  const int nb = c.chol_nb;
+  const int nb = fla_chol_nb(n);
   const size_t as_bytes = ((size_t)n * n * sizeof(float) + 255) & ~(size_t)255;
   const size_t p_bytes = ((size_t)lp_panel_elems(n, nb, bf16) * (bf16 ? 2 : 4) + 255) &
                          ~(size_t)255;

@@ -223,6 +223,16 @@ extern "C" int fla_hip_set_blocksizes(int chol_nb, int lu_nb) {
   return FLA_HIP_OK;
 }
 
+int fla_chol_nb(int n) {
+  if (g_ctx.chol_nb > 0) return g_ctx.chol_nb;
+  int nb = (n / 32) / 128 * 128;
+  if (nb < 256) nb = 256;
+  if (nb > 1024) nb = 1024;
+  return nb;
+}
+
+extern "C" int fla_hip_chol_blocksize(int n) { return fla_chol_nb(n); }
+
 extern "C" int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb) {
   if (chol_nb) *chol_nb = g_ctx.chol_nb;
   if (lu_nb) *lu_nb = g_ctx.lu_nb;
