@@ -92,6 +92,16 @@ def lib():
             "fla_hip_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
             "fla_hip_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
             "fla_hip_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
+            "fla_hip_memcpy2d": (C.c_int, [vp, C.c_size_t, vp, C.c_size_t, C.c_size_t,
+                                           C.c_size_t, C.c_int]),
+            "fla_hip_convert_d2s": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, C.c_int]),
+            "fla_hip_batched_tune": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, ip, dp]),
+            "fla_hip_batched_set_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "fla_hip_batched_set_graph": (C.c_int, [C.c_int]),
+            "fla_hip_set_gemm_config": (C.c_int, [C.c_int]),
+            "fla_hip_set_gemm_group": (C.c_int, [C.c_int]),
+            "fla_hip_set_reserved_cus": (C.c_int, [C.c_int]),
+            "fla_hip_mp_set_panel": (C.c_int, [C.c_int]),
             "fla_hip_error_string": (C.c_char_p, [C.c_int]),
             "fla_hip_dgemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                         vp, C.c_int, vp, C.c_int, C.c_double, vp, C.c_int]),
