@@ -145,10 +145,16 @@ int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* 
    [3] FP64 products with A, [4] ||A||_inf, [5] GMRES iterations, [6] the
    reduced-precision factorization's info. */
 int fla_hip_mp_last_stats(double* out, int nout);
-/* Panel solve of the reduced-precision factor: 0 (default) blocked
-   substitution, 1 multiplication by the inverse of each 128 x 128 diagonal
+/* Panel solve of the reduced-precision factor: 0 blocked substitution,
+   1 (default) multiplication by the inverse of each 128 x 128 diagonal
    factor. */
 int fla_hip_mp_set_panel(int inverse);
+/* CUs left out of the reduced-precision factor's update stream for its panel
+   chain (default 64; 0 = no mask). */
+int fla_hip_mp_set_reserved_cus(int reserved);
+/* Outer block of the reduced-precision factor (multiple of 128; 0 = the
+   FP64 Cholesky's choice for the order). */
+int fla_hip_mp_set_blocksize(int nb);
 
 /* ---- batched small factorizations and solves (no counterpart in the
    reference: thousands of independent matrices of uneven sizes in one

@@ -102,6 +102,8 @@ def lib():
             "fla_hip_set_gemm_group": (C.c_int, [C.c_int]),
             "fla_hip_set_reserved_cus": (C.c_int, [C.c_int]),
             "fla_hip_mp_set_panel": (C.c_int, [C.c_int]),
+            "fla_hip_mp_set_reserved_cus": (C.c_int, [C.c_int]),
+            "fla_hip_mp_set_blocksize": (C.c_int, [C.c_int]),
             "fla_hip_error_string": (C.c_char_p, [C.c_int]),
             "fla_hip_dgemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                         vp, C.c_int, vp, C.c_int, C.c_double, vp, C.c_int]),

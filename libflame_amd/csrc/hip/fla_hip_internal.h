@@ -32,6 +32,9 @@ struct FlaHipCtx {
   int num_cus = 0;
   hipStream_t s_update = nullptr;  // trailing updates (and the API's default stream)
   hipStream_t s_panel = nullptr;   // panel factorizations (look-ahead)
+  // Trailing updates of the reduced-precision factor: their panel chain, not
+  // the update, is the critical path, so this stream leaves more CUs free.
+  hipStream_t s_update_lp = nullptr;
   static constexpr int kEvents = 8;
   hipEvent_t ev[kEvents] = {};
   // Grow-only device workspaces: slot 0 general scratch (dtrsm, dlaswp),
@@ -44,6 +47,7 @@ struct FlaHipCtx {
   // 128 x 128 FP64 block: inverse of the current Cholesky diagonal block.
   double* dlinv = nullptr;
   int chol_nb = 0;  // 0: chosen from n (fla_chol_nb)
+  int mp_nb = 0;    // reduced-precision factor block; 0: fla_chol_nb
   int lu_nb = 128;
 };
 

@@ -340,10 +340,12 @@ inline unsigned nblocks(long total) { return (unsigned)((total + 255) / 256); }
 
 // ---- reduced-precision factorization ------------------------------------------
 
-// Panel solve of the reduced-precision factor: blocked substitution
-// (default; backward stable for any triangle) or the product with the
-// inverse of each 128 x 128 diagonal factor.
-bool g_mp_panel_inverse = false;
+// Panel solve of the reduced-precision factor: the product with the inverse
+// of each 128 x 128 diagonal factor (default: an MFMA product instead of a
+// latency-bound substitution on the panel chain; the diagonal factors of a
+// matrix that the reduced precision can factor are well conditioned), or
+// blocked substitution (backward stable for any triangle).
+bool g_mp_panel_inverse = true;
 
 struct LpFactor {
   float* As = nullptr;      // FP32 trailing matrix (lower), ld n
@@ -377,7 +379,7 @@ int lp_panel(hipStream_t s, int n, LpFactor& f, double* L, long ldl, int k, int 
 // s_panel) overlaps the reduced-precision update of step k (s_update).
 int lp_chol_lower(int n, LpFactor& f, double* L, long ldl, bool bf16, int nb, int* dinfo) {
   FlaHipCtx& c = fla_ctx();
-  hipStream_t sp = c.s_panel, su = c.s_update;
+  hipStream_t sp = c.s_panel, su = c.s_update_lp;
   int evi = 0;
   auto next_event = [&]() {
     hipEvent_t e = c.ev[evi];
@@ -385,8 +387,9 @@ int lp_chol_lower(int n, LpFactor& f, double* L, long ldl, bool bf16, int nb, in
     return e;
   };
   hipEvent_t start = next_event();
-  FLA_HIP_CHECK(hipEventRecord(start, su));
+  FLA_HIP_CHECK(hipEventRecord(start, c.s_update));
   FLA_HIP_CHECK(hipStreamWaitEvent(sp, start, 0));
+  FLA_HIP_CHECK(hipStreamWaitEvent(su, start, 0));
   int slot = 0;
   FLA_TRY(lp_panel(sp, n, f, L, ldl, 0, n < nb ? n : nb, bf16, slot, dinfo));
   for (int k = 0; k < n;) {
@@ -418,7 +421,10 @@ int lp_chol_lower(int n, LpFactor& f, double* L, long ldl, bool bf16, int nb, in
   }
   hipEvent_t fin = next_event();
   FLA_HIP_CHECK(hipEventRecord(fin, sp));
-  FLA_HIP_CHECK(hipStreamWaitEvent(su, fin, 0));
+  FLA_HIP_CHECK(hipStreamWaitEvent(c.s_update, fin, 0));
+  hipEvent_t fin_u = next_event();
+  FLA_HIP_CHECK(hipEventRecord(fin_u, su));
+  FLA_HIP_CHECK(hipStreamWaitEvent(c.s_update, fin_u, 0));
   return FLA_HIP_OK;
 }
 
@@ -431,7 +437,7 @@ int mp_chol(int upper, int n, const double* A, long lda, bool bf16, double* L, l
             int* info, int* ovf) {
   FlaHipCtx& c = fla_ctx();
   hipStream_t s = c.s_update;
-  const int nb = fla_chol_nb(n);
+  const int nb = c.mp_nb > 0 ? c.mp_nb : fla_chol_nb(n);
   const size_t as_bytes = ((size_t)n * n * sizeof(float) + 255) & ~(size_t)255;
   const size_t p_bytes = ((size_t)lp_panel_elems(n, nb, bf16) * (bf16 ? 2 : 4) + 255) &
                          ~(size_t)255;

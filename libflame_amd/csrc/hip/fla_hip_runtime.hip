@@ -20,25 +20,32 @@ FlaHipCtx& fla_ctx() { return g_ctx; }
 // single-workgroup panel kernel that needs a whole CU's LDS would otherwise
 // wait for a CU to drain completely.
 static int g_reserved_cus = 8;
+// The reduced-precision factor's update stream leaves 64 CUs to its panels
+// (measured on MI355X at n = 16384).
+static int g_lp_reserved_cus = 64;
 
-static int create_update_stream(int reserved) {
+static int create_masked_stream(hipStream_t* s, int reserved) {
   if (reserved <= 0 || reserved >= g_ctx.num_cus) {
-    FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
     return FLA_HIP_OK;
   }
   const int words = (g_ctx.num_cus + 31) / 32;
   uint32_t mask[16] = {};
   if (words > 16) {
-    FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
     return FLA_HIP_OK;
   }
   const int use = g_ctx.num_cus - reserved;
   for (int c = 0; c < use; ++c) mask[c / 32] |= 1u << (c % 32);
-  if (hipExtStreamCreateWithCUMask(&g_ctx.s_update, (uint32_t)words, mask) != hipSuccess) {
+  if (hipExtStreamCreateWithCUMask(s, (uint32_t)words, mask) != hipSuccess) {
     (void)hipGetLastError();
-    FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_update, hipStreamNonBlocking));
+    FLA_HIP_CHECK(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
   }
   return FLA_HIP_OK;
+}
+
+static int create_update_stream(int reserved) {
+  return create_masked_stream(&g_ctx.s_update, reserved);
 }
 
 int fla_ensure_ready() {
@@ -64,6 +71,7 @@ extern "C" int fla_hip_init(void) {
   g_ctx.num_cus = prop.multiProcessorCount;
   // Default-priority streams only.
   FLA_TRY(create_update_stream(g_reserved_cus));
+  FLA_TRY(create_masked_stream(&g_ctx.s_update_lp, g_lp_reserved_cus));
   FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_panel, hipStreamNonBlocking));
   for (int i = 0; i < FlaHipCtx::kEvents; ++i)
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
@@ -87,6 +95,7 @@ extern "C" int fla_hip_finalize(void) {
   g_ctx.dlinv = nullptr;
   for (int i = 0; i < FlaHipCtx::kEvents; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev[i]));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
+  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update_lp));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_panel));
   g_ctx.ready = false;
   return FLA_HIP_OK;
@@ -95,6 +104,7 @@ extern "C" int fla_hip_finalize(void) {
 extern "C" int fla_hip_synchronize(void) {
   FLA_TRY(fla_ensure_ready());
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_panel));
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update_lp));
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
   return FLA_HIP_OK;
 }
@@ -336,4 +346,21 @@ extern "C" int fla_hip_set_reserved_cus(int reserved) {
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
   g_ctx.s_update = nullptr;
   return create_update_stream(reserved);
+}
+
+extern "C" int fla_hip_mp_set_reserved_cus(int reserved) {
+  if (reserved < 0) return FLA_HIP_EARG;
+  g_lp_reserved_cus = reserved;
+  if (!g_ctx.ready) return FLA_HIP_OK;
+  FLA_HIP_CHECK(hipDeviceSynchronize());
+  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update_lp));
+  g_ctx.s_update_lp = nullptr;
+  return create_masked_stream(&g_ctx.s_update_lp, reserved);
+}
+
+extern "C" int fla_hip_mp_set_blocksize(int nb) {
+  if (nb < 0 || nb % 128 != 0) return FLA_HIP_EARG;
+  FLA_TRY(fla_ensure_ready());
+  g_ctx.mp_nb = nb;
+  return FLA_HIP_OK;
 }

@@ -59,7 +59,7 @@ def substitution_panel():
     these integer matrices have astronomically large inverses."""
     fla.lib().fla_hip_mp_set_panel(0)
     yield
-    fla.lib().fla_hip_mp_set_panel(0)
+    fla.lib().fla_hip_mp_set_panel(1)
 
 
 @pytest.mark.parametrize("precision", [fla.FLA_MP_FP32, fla.FLA_MP_BF16])
@@ -89,7 +89,7 @@ def test_mp_chol_panel_modes_agree(orc, inverse):
     try:
         st, info, Lf = mp_chol(A, fla.FLA_MP_FP32)
     finally:
-        fla.lib().fla_hip_mp_set_panel(0)
+        fla.lib().fla_hip_mp_set_panel(1)
     assert st == fla.FLA_HIP_OK and info == fla.FLA_SUCCESS
     assert np.linalg.norm(A - Lf @ Lf.T, 1) / np.linalg.norm(A, 1) <= 8 * 2.0 ** -24
 
