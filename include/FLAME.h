@@ -24,6 +24,7 @@
  *   dpotrf_ / dgetrf_          src/map/lapack2flamec/FLA_potrf.c:31-105, FLA_getrf.c:32-117
  *   dpotrs_ / dgetrs_          src/map/common/lapacksrc/f2c/dpotrs.c:102, dgetrs.c:115
  *   dsposv_                    src/map/common/lapacksrc/f2c/dsposv.c:192-415
+ *   dsgesv_                    src/map/common/lapacksrc/f2c/dsgesv.c:189-407
  */
 #ifndef LIBFLAME_AMD_FLAME_H
 #define LIBFLAME_AMD_FLAME_H
@@ -244,6 +245,8 @@ int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* 
 int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b,
             int* ldb, int* info);
 int dsposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, double* x,
+            int* ldx, double* work, float* swork, int* iter, int* info);
+int dsgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb, double* x,
             int* ldx, double* work, float* swork, int* iter, int* info);
 
 #ifdef __cplusplus

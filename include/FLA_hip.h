@@ -107,7 +107,8 @@ int fla_hip_set_reserved_cus(int reserved);
    algorithmic flops and the elapsed milliseconds.
    kind 0 = Cholesky trailing update (lower SYRK), 1 = LU trailing update
    (GEMM), 2 = LU panel, 3 = Cholesky panel chain, 4 = reduced-precision
-   Cholesky trailing update (FP32 / BF16 MFMA). */
+   Cholesky trailing update (FP32 / BF16 MFMA), 5 = reduced-precision LU
+   trailing update. */
 int fla_hip_stats_enable(int on);
 int fla_hip_stats_reset(void);
 int fla_hip_stats_get(int kind, double* flops, double* ms, long* launches);
@@ -140,7 +141,25 @@ int fla_hip_mp_chol(int uplo, int n, const double* A, int lda, int precision, do
    opts may be NULL (FP32, classic refinement, 30 steps). */
 int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* B, int ldb,
                    double* X, int ldx, const fla_hip_mp_opts* opts, int* iter, int* info);
-/* Timings and counters of the last fla_hip_dsposv: [0] factor ms (incl.
+/* Reduced-precision LU with partial pivoting of the general n x n A
+   (dsgesv_'s factorization, src/map/common/lapacksrc/f2c/dsgesv.c:314):
+   FP32 trailing matrix, FP64 panels (the LU panel kernel) and
+   U12 := inv(L11) A12 in FP64, A22 -= L21 U12 in FP32 or BF16 on MFMA.
+   F receives the factors (FP64 values, unit L below the diagonal), p the
+   FLAME-native pivots (device, n ints).  *info = FLA_SUCCESS or the 0-based
+   column of the first null pivot.  FLA_HIP_ERANGE if A overflows FP32. */
+int fla_hip_mp_lu(int n, const double* A, int lda, int precision, double* F, int ldf, int* p,
+                  int* info);
+/* Solves A X = B for general A by the reduced-precision LU and refinement
+   with an FP64 residual: dsgesv_ (src/map/common/lapacksrc/f2c/dsgesv.c:189)
+   with the iteration codes, limit and FP64 fall-back of fla_hip_dsposv.  p
+   (device, n ints) receives the FLAME-native pivots of the factorization
+   used: the reduced-precision one, or after a fall-back the FP64 dgetrf's,
+   with A then overwritten by the FP64 factors.  *info: 0, or k > 0 when
+   U(k-1, k-1) of the FP64 fall-back is exactly zero. */
+int fla_hip_dsgesv(int n, int nrhs, double* A, int lda, int* p, const double* B, int ldb,
+                   double* X, int ldx, const fla_hip_mp_opts* opts, int* iter, int* info);
+/* Timings and counters of the last fla_hip_dsposv / fla_hip_dsgesv: [0] factor ms (incl.
    conversion and norm), [1] refinement ms, [2] preconditioner solves,
    [3] FP64 products with A, [4] ||A||_inf, [5] GMRES iterations, [6] the
    reduced-precision factorization's info. */
@@ -222,6 +241,10 @@ int fla_hip_gram_lower(int n, int k, const double* B, int ldb, double scale,
    nrefl Householder reflectors with N(0,1) vectors (full matrix written). */
 int fla_hip_gen_spd_reflectors(unsigned long long seed, int n, int nrefl, double kappa,
                                double* A, int lda);
+/* General A = Q1 diag(s) Q2' with the same s, Q1 a product of nrefl
+   reflectors drawn from seed and Q2 an independent product drawn from seed2. */
+int fla_hip_gen_ge_reflectors(unsigned long long seed, unsigned long long seed2, int n, int nrefl,
+                              double kappa, double* A, int lda);
 /* S := float(A), m x n (no overlap). */
 int fla_hip_convert_d2s(int m, int n, const double* A, int lda, float* S, int lds);
 /* A(i,:) *= 10^(-i/max(m,n)) */

@@ -85,6 +85,8 @@ def lib():
             "dgetrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, ip, dp, ip, ip]),
             "dsposv_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, dp, ip, dp,
                                   C.POINTER(C.c_float), ip, ip]),
+            "dsgesv_": (C.c_int, [ip, ip, dp, ip, ip, dp, ip, dp, ip, dp,
+                                  C.POINTER(C.c_float), ip, ip]),
             "fla_hip_init": (C.c_int, []), "fla_hip_finalize": (C.c_int, []),
             "fla_hip_synchronize": (C.c_int, []),
             "fla_hip_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
@@ -135,6 +137,11 @@ def lib():
                                           ip]),
             "fla_hip_dsposv": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int,
                                          vp, C.c_int, C.POINTER(MpOpts), ip, ip]),
+            "fla_hip_dsgesv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int,
+                                         vp, C.c_int, C.POINTER(MpOpts), ip, ip]),
+            "fla_hip_mp_lu": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp, ip]),
+            "fla_hip_gen_ge_reflectors": (C.c_int, [C.c_ulonglong, C.c_ulonglong, C.c_int,
+                                                    C.c_int, C.c_double, vp, C.c_int]),
             "fla_hip_mp_last_stats": (C.c_int, [dp, C.c_int]),
             "fla_hip_dchol_adjoint": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int]),
             "fla_hip_dchol_solve_adjoint": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, C.c_int,
@@ -174,6 +181,21 @@ def dsposv(uplo, A, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_
     it, info = C.c_int(), C.c_int()
     check(lib().fla_hip_dsposv(uplo, A.m, B.n, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld,
                                C.byref(o), C.byref(it), C.byref(info)), "fla_hip_dsposv")
+    st = (C.c_double * 8)()
+    check(lib().fla_hip_mp_last_stats(st, 8))
+    keys = ["factor_ms", "refine_ms", "prec_solves", "matvecs", "anorm", "gmres_iters",
+            "lp_info"]
+    return it.value, info.value, {k: st[i] for i, k in enumerate(keys)}
+
+
+def dsgesv(A, p, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_iter=30,
+           gmres_restart=20, gmres_tol=1e-4):
+    """fla_hip_dsgesv on device matrices (p: device int vector of n); returns
+    (iter, info, stats dict)."""
+    o = MpOpts(precision, refinement, max_iter, gmres_restart, gmres_tol)
+    it, info = C.c_int(), C.c_int()
+    check(lib().fla_hip_dsgesv(A.m, B.n, A.ptr, A.ld, p.ptr, B.ptr, B.ld, X.ptr, X.ld,
+                               C.byref(o), C.byref(it), C.byref(info)), "fla_hip_dsgesv")
     st = (C.c_double * 8)()
     check(lib().fla_hip_mp_last_stats(st, 8))
     keys = ["factor_ms", "refine_ms", "prec_solves", "matvecs", "anorm", "gmres_iters",
@@ -324,6 +346,7 @@ def set_blocksizes(chol_nb=0, lu_nb=0):
 
 STATS_CHOL_UPDATE, STATS_LU_UPDATE, STATS_LU_PANEL, STATS_CHOL_PANEL = 0, 1, 2, 3
 STATS_MP_UPDATE = 4
+STATS_MP_LU_UPDATE = 5
 
 
 def stats_enable(on=True):

@@ -429,12 +429,12 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
 // at A: new row q := old row orig, through registers (all reads of a
 // workgroup's columns precede its writes).
 constexpr int GC = 16;  // columns per workgroup
-__global__ __launch_bounds__(256) void lu_row_gather_kernel(int ncols, double* __restrict__ A,
-                                                            long lda,
+template <typename T>
+__global__ __launch_bounds__(256) void lu_row_gather_kernel(int ncols, T* __restrict__ A, long lda,
                                                             const int* __restrict__ moved) {
   const int cnt = moved[0];
   const int c0 = blockIdx.x * GC;
-  double val[GC];
+  T val[GC];
   const int t = threadIdx.x;
   // cnt <= 2 * PW = 256: one pair per thread.
   int dst = -1, src = -1;
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(256) void lu_row_gather_kernel(int ncols, double* _
   if (src >= 0) {
 #pragma unroll
     for (int c = 0; c < GC; ++c)
-      val[c] = (c0 + c < ncols) ? A[(long)src + (long)(c0 + c) * lda] : 0.0;
+      val[c] = (c0 + c < ncols) ? A[(long)src + (long)(c0 + c) * lda] : T(0);
   }
   __syncthreads();
   if (dst >= 0) {
@@ -528,8 +528,8 @@ static int lu_panel(hipStream_t s, int M, int W, double* Ap, long lda, int k, in
 
 static int lu_gather(hipStream_t s, int ncols, double* A, long lda, const int* moved) {
   if (ncols <= 0) return FLA_HIP_OK;
-  hipLaunchKernelGGL(lu_row_gather_kernel, dim3((ncols + GC - 1) / GC), dim3(256), 0, s, ncols, A,
-                     lda, moved);
+  hipLaunchKernelGGL(lu_row_gather_kernel<double>, dim3((ncols + GC - 1) / GC), dim3(256), 0, s,
+                     ncols, A, lda, moved);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
@@ -626,5 +626,45 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
   fla_stats_harvest();
   if (h[1] != 0) return FLA_HIP_ESPIN;
   *info = h[0];
+  return FLA_HIP_OK;
+}
+
+// ---- building blocks of the reduced-precision LU (fla_hip_mixed.hip) ----------
+
+static LuBufs g_mp_bufs[2];
+
+int fla_lu_panel_setup(int m, int* derr) {
+  FlaHipCtx& c = fla_ctx();
+  const int Pmax = (m + ROWS_PER_WG - 1) / ROWS_PER_WG;
+  if (Pmax > c.num_cus) {
+    std::fprintf(stderr, "libflame_amd: the LU panel supports m <= %d on this device\n",
+                 c.num_cus * ROWS_PER_WG);
+    return FLA_HIP_EARG;
+  }
+  const size_t set = align256(lu_set_bytes(Pmax > 0 ? Pmax : 1));
+  void* sp = nullptr;
+  FLA_TRY(lu_scratch(2 * set, &sp));
+  g_mp_bufs[0] = lu_carve(static_cast<char*>(sp), Pmax);
+  g_mp_bufs[1] = lu_carve(static_cast<char*>(sp) + set, Pmax);
+  g_mp_bufs[0].cm.err = derr;
+  g_mp_bufs[1].cm.err = derr;
+  return FLA_HIP_OK;
+}
+
+int fla_lu_panel_run(hipStream_t s, int set, int M, int W, double* Ap, long lda, int k, int* piv,
+                     int* dinfo) {
+  if (W > PW) return FLA_HIP_EARG;
+  return lu_panel(s, M, W, Ap, lda, k, piv, dinfo, g_mp_bufs[set & 1]);
+}
+
+int fla_lu_gather_rows(hipStream_t s, int set, int ncols, double* A, long lda) {
+  return lu_gather(s, ncols, A, lda, g_mp_bufs[set & 1].moved);
+}
+
+int fla_lu_gather_rows_f(hipStream_t s, int set, int ncols, float* A, long lda) {
+  if (ncols <= 0) return FLA_HIP_OK;
+  hipLaunchKernelGGL(lu_row_gather_kernel<float>, dim3((ncols + GC - 1) / GC), dim3(256), 0, s,
+                     ncols, A, lda, g_mp_bufs[set & 1].moved);
+  FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

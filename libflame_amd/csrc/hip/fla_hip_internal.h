@@ -184,5 +184,18 @@ int chol_panel_f64(hipStream_t s, int n, double* A, long lda, int k, int kb, int
 // The same with the panel solve done as a product with the explicit inverse
 // of each 128 x 128 diagonal factor (fewer launches, preconditioner quality).
 int chol_panel_f64_inv(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info);
+// LU panel building blocks (fla_hip_getrf.hip) for the reduced-precision LU:
+// two alternating buffer sets for panel launches on matrices of up to m
+// rows; derr is the device int set when a bounded wait expires.
+int fla_lu_panel_setup(int m, int* derr);
+// FP64 LU with partial pivoting of the M x W panel at Ap (W <= 128), whose
+// first column is column k of the matrix: pivots into piv[k..k+W), the first
+// null pivot's column into *dinfo; the row moves are kept in buffer set `set`.
+int fla_lu_panel_run(hipStream_t s, int set, int M, int W, double* Ap, long lda, int k, int* piv,
+                     int* dinfo);
+// Applies buffer set `set`'s row moves to ncols columns of the block whose
+// first row is the panel's first row (FP64 / FP32).
+int fla_lu_gather_rows(hipStream_t s, int set, int ncols, double* A, long lda);
+int fla_lu_gather_rows_f(hipStream_t s, int set, int ncols, float* A, long lda);
 // In-place exchange of the strictly lower and strictly upper triangles.
 int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

@@ -1,7 +1,8 @@
-// Reduced-precision trailing updates of the mixed-precision Cholesky:
+// Reduced-precision trailing updates of the mixed-precision factorizations:
 //   C (FP32, column-major) += alpha * P * Q^T
-// with P, Q rows of the factored panel rounded to BF16 (v_mfma_f32_16x16x32_bf16)
-// or FP32 (v_mfma_f32_16x16x4f32), FP32 accumulation in both cases.
+// with P, Q rounded to BF16 (v_mfma_f32_16x16x32_bf16) or FP32
+// (v_mfma_f32_16x16x4f32), FP32 accumulation in both cases.  Cholesky: P and
+// Q are rows of the factored panel L21; LU: P = L21 and Q = U12^T.
 //
 // The panel is first packed into "fragment order": for each 16-row tile rt
 // and k-block kb (32 k for BF16, 16 k for FP32) one 1 KB block holds, for
@@ -13,8 +14,8 @@
 // spare tile row) and k to a whole block.
 //
 // Per-panel scaling: the packed values are multiplied by 2^-e, with e the
-// binary exponent of the panel's largest magnitude (so they lie in [-1, 1));
-// the update multiplies its FP32 result by 2^(2e).  Both factors are powers of
+// binary exponent of the operand's largest magnitude (so they lie in
+// [-1, 1)); the update multiplies its FP32 result by 2^(e_P + e_Q).  Both factors are powers of
 // two, so the scaling is exact; it keeps the packed operands in the normal
 // range of the narrow format.
 #include "fla_hip_internal.h"
@@ -59,9 +60,12 @@ __device__ __forceinline__ float pow2_scale(const unsigned* amax_bits) {
   return ldexpf(1.0f, -e);
 }
 
+// Element (row, kk) of the operand is P[row + kk*ldp], or P[kk + row*ldp]
+// when `trans` (the rows of U12^T are the columns of U12).
 template <bool BF16>
-__global__ void lp_pack_kernel(int m, int k, const double* __restrict__ P, long ldp, int rtiles,
-                               int nkb, const unsigned* __restrict__ amax_bits, void* out) {
+__global__ void lp_pack_kernel(int m, int k, const double* __restrict__ P, long ldp, int trans,
+                               int rtiles, int nkb, const unsigned* __restrict__ amax_bits,
+                               void* out) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   const long total = (long)rtiles * nkb * 64;
   if (idx >= total) return;
@@ -75,7 +79,8 @@ __global__ void lp_pack_kernel(int m, int k, const double* __restrict__ P, long 
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int kk = kb * 32 + 8 * (lane >> 4) + e;
-      const float x = (row < m && kk < k) ? (float)P[(long)row + (long)kk * ldp] * sc : 0.0f;
+      const long at = trans ? (long)kk + (long)row * ldp : (long)row + (long)kk * ldp;
+      const float x = (row < m && kk < k) ? (float)P[at] * sc : 0.0f;
       v[e] = (__bf16)x;
     }
     reinterpret_cast<bf8*>(out)[idx] = v;
@@ -84,7 +89,8 @@ __global__ void lp_pack_kernel(int m, int k, const double* __restrict__ P, long 
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int kk = kb * 16 + 4 * e + (lane >> 4);
-      v[e] = (row < m && kk < k) ? (float)P[(long)row + (long)kk * ldp] * sc : 0.0f;
+      const long at = trans ? (long)kk + (long)row * ldp : (long)row + (long)kk * ldp;
+      v[e] = (row < m && kk < k) ? (float)P[at] * sc : 0.0f;
     }
     reinterpret_cast<f4*>(out)[idx] = v;
   }
@@ -100,7 +106,8 @@ __global__ __launch_bounds__(256) void lp_update_kernel(int m, int n, int nkb,
                                                         const void* __restrict__ Pv, int prt0,
                                                         const void* __restrict__ Qv, int qrt0,
                                                         float alpha,
-                                                        const unsigned* __restrict__ amax_bits,
+                                                        const unsigned* __restrict__ amax_p,
+                                                        const unsigned* __restrict__ amax_q,
                                                         float* __restrict__ C, long ldc, int Tm,
                                                         int Tn, int group,
                                                         const int* __restrict__ abort) {
@@ -171,8 +178,7 @@ __global__ __launch_bounds__(256) void lp_update_kernel(int m, int n, int nkb,
       for (int b = 0; b < 4; ++b) qb[b] = qn[b];
     }
   }
-  const float s = pow2_scale(amax_bits);
-  const float f = alpha / (s * s);
+  const float f = alpha / (pow2_scale(amax_p) * pow2_scale(amax_q));
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int i = i0 + a * 16 + (lane & 15);
@@ -211,18 +217,18 @@ int launch_lp_absmax(hipStream_t s, int m, int k, const double* P, long ldp, uns
 }
 
 int launch_lp_pack(hipStream_t s, bool bf16, int m, int k, const double* P, long ldp,
-                   const unsigned* amax_bits, void* out) {
+                   const unsigned* amax_bits, void* out, bool trans) {
   const int kbw = bf16 ? 32 : 16;
   const int rtiles = (int)(((long)(m + 127) / 128 + 1) * 8);
   const int nkb = (k + kbw - 1) / kbw;
   const long total = (long)rtiles * nkb * 64;
   const unsigned blocks = (unsigned)((total + 255) / 256);
   if (bf16)
-    hipLaunchKernelGGL(lp_pack_kernel<true>, dim3(blocks), dim3(256), 0, s, m, k, P, ldp, rtiles,
-                       nkb, amax_bits, out);
+    hipLaunchKernelGGL(lp_pack_kernel<true>, dim3(blocks), dim3(256), 0, s, m, k, P, ldp,
+                       trans ? 1 : 0, rtiles, nkb, amax_bits, out);
   else
     hipLaunchKernelGGL(lp_pack_kernel<false>, dim3(blocks), dim3(256), 0, s, m, k, P, ldp,
-                       rtiles, nkb, amax_bits, out);
+                       trans ? 1 : 0, rtiles, nkb, amax_bits, out);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
@@ -232,8 +238,8 @@ int launch_lp_pack(hipStream_t s, bool bf16, int m, int k, const double* P, long
 // multiples of 16; the packed panels must hold prow0 + 128*ceil(m/128) and
 // qrow0 + 128*ceil(n/128) rows (lp_panel_elems pads for that).
 int launch_lp_update(hipStream_t s, bool bf16, bool lower, int m, int n, int k, const void* P,
-                     int prow0, const void* Q, int qrow0, float alpha, const unsigned* amax_bits,
-                     float* C, long ldc, const int* abort_flag) {
+                     int prow0, const void* Q, int qrow0, float alpha, const unsigned* amax_p,
+                     const unsigned* amax_q, float* C, long ldc, const int* abort_flag) {
   if (m <= 0 || n <= 0 || k <= 0) return FLA_HIP_OK;
   if ((prow0 & 15) || (qrow0 & 15)) return FLA_HIP_EARG;
   const int kbw = bf16 ? 32 : 16;
@@ -245,16 +251,16 @@ int launch_lp_update(hipStream_t s, bool bf16, bool lower, int m, int n, int k, 
   const dim3 grid((unsigned)nblk), block(256);
   if (bf16 && lower)
     hipLaunchKernelGGL((lp_update_kernel<true, true>), grid, block, 0, s, m, n, nkb, P, prow0 / 16,
-                       Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
+                       Q, qrow0 / 16, alpha, amax_p, amax_q, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   else if (bf16)
     hipLaunchKernelGGL((lp_update_kernel<true, false>), grid, block, 0, s, m, n, nkb, P,
-                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
+                       prow0 / 16, Q, qrow0 / 16, alpha, amax_p, amax_q, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   else if (lower)
     hipLaunchKernelGGL((lp_update_kernel<false, true>), grid, block, 0, s, m, n, nkb, P,
-                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
+                       prow0 / 16, Q, qrow0 / 16, alpha, amax_p, amax_q, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   else
     hipLaunchKernelGGL((lp_update_kernel<false, false>), grid, block, 0, s, m, n, nkb, P,
-                       prow0 / 16, Q, qrow0 / 16, alpha, amax_bits, C, ldc, Tm, Tn, g_tile_group, abort_flag);
+                       prow0 / 16, Q, qrow0 / 16, alpha, amax_p, amax_q, C, ldc, Tm, Tn, g_tile_group, abort_flag);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

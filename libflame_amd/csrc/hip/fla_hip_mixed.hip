@@ -17,6 +17,13 @@
 // M = L L^T), GMRES-IR solves A d = r by left-preconditioned GMRES in FP64
 // with M^-1 as the preconditioner, which converges where kappa(A) * u_factor
 // is too large for classic refinement.
+//
+// The general solve follows dsgesv_ (src/map/common/lapacksrc/f2c/dsgesv.c:
+// 189-407) the same way: a reduced-precision LU with partial pivoting (FP64
+// panels by the LU panel kernel, FP32 trailing matrix, row interchanges and
+// U12 := inv(L11) A12 in FP64, A22 -= L21 U12 on MFMA in FP32 or BF16), an
+// FP64 residual with the general A, the same stopping rule and fall-back to
+// the FP64 dgetrf / dgetrs.
 #include "fla_hip_internal.h"
 
 #include <cfloat>
@@ -27,10 +34,10 @@
 long lp_panel_elems(int m, int k, bool bf16);
 int launch_lp_absmax(hipStream_t s, int m, int k, const double* P, long ldp, unsigned* out);
 int launch_lp_pack(hipStream_t s, bool bf16, int m, int k, const double* P, long ldp,
-                   const unsigned* amax_bits, void* out);
+                   const unsigned* amax_bits, void* out, bool trans);
 int launch_lp_update(hipStream_t s, bool bf16, bool lower, int m, int n, int k, const void* P,
-                     int prow0, const void* Q, int qrow0, float alpha, const unsigned* amax_bits,
-                     float* C, long ldc, const int* abort_flag);
+                     int prow0, const void* Q, int qrow0, float alpha, const unsigned* amax_p,
+                     const unsigned* amax_q, float* C, long ldc, const int* abort_flag);
 
 namespace {
 
@@ -84,15 +91,60 @@ __global__ void lat2s_kernel(int n, const double* __restrict__ A, long lda, int 
   if (bad) atomicOr(ovf, 1);
 }
 
-// Ld := double(As) on the lower part of an m x kb panel whose top-left
-// element is on the diagonal; resets the panel's absmax slot.
+// Ld := double(As) on an m x kb block -- only its lower part (i >= j) when
+// `lower` (a Cholesky panel whose top-left element is on the diagonal); resets
+// the absmax slot of the operand packed from it.
 __global__ void s2d_panel_kernel(int m, int kb, const float* __restrict__ As, long ldas,
-                                 double* __restrict__ Ld, long ldl, unsigned* amax) {
+                                 double* __restrict__ Ld, long ldl, unsigned* amax, int lower) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   if (idx == 0) *amax = 0u;
   if (idx >= (long)m * kb) return;
   const int i = (int)(idx % m), j = (int)(idx / m);
-  if (i >= j) Ld[(long)i + (long)j * ldl] = (double)As[(long)i + (long)j * ldas];
+  if (!lower || i >= j) Ld[(long)i + (long)j * ldl] = (double)As[(long)i + (long)j * ldas];
+}
+
+// As := float(A) for a general m x n matrix; *ovf = 1 when an entry lies
+// outside [-FLT_MAX, FLT_MAX] (dlag2s_).
+__global__ void ge2s_kernel(int m, int n, const double* __restrict__ A, long lda,
+                            float* __restrict__ As, long ldas, int* __restrict__ ovf) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  const double v = A[(long)i + (long)j * lda];
+  if (v < -FLT_MAX || v > FLT_MAX) atomicOr(ovf, 1);
+  As[(long)i + (long)j * ldas] = (float)v;
+}
+
+// Partial row sums of |A| for a general A: Pp[split * n + i] over the
+// columns [split * jchunk, +jchunk) (one thread per row: coalesced).
+__global__ void geinf_partial_kernel(int n, const double* __restrict__ A, long lda, int jchunk,
+                                     double* __restrict__ Pp) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int j0 = blockIdx.y * jchunk;
+  const int j1 = (j0 + jchunk) < n ? (j0 + jchunk) : n;
+  double s = 0.0;
+  for (int j = j0; j < j1; ++j) s += fabs(A[(long)i + (long)j * lda]);
+  Pp[(long)blockIdx.y * n + i] = s;
+}
+
+// Pp[i] := sum over the splits (fixed order).
+__global__ void geinf_rows_kernel(int n, int nsplit, double* __restrict__ Pp) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int p = 0; p < nsplit; ++p) s += Pp[(long)p * n + i];
+  Pp[i] = s;
+}
+
+// dst(q, :) := src(perm[q], :) on an n x nrhs block.
+__global__ void perm_rows_kernel(int n, int nrhs, const int* __restrict__ perm,
+                                 const double* __restrict__ src, long lds, double* __restrict__ dst,
+                                 long ldd) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int q = (int)(idx % n), c = (int)(idx / n);
+  dst[(long)q + (long)c * ldd] = src[(long)perm[q] + (long)c * lds];
 }
 
 // ---- symmetric products and norms -------------------------------------------
@@ -103,14 +155,15 @@ __device__ __forceinline__ double sym_at(const double* A, long lda, int upper, i
   return direct ? A[(long)i + (long)j * lda] : A[(long)j + (long)i * lda];
 }
 
-// Partial products of R := A X for a symmetric A (block row I, blocks
-// J in [J0, J1)): Rp[(split)*n*16 + i + col*n].  4 waves x (32 rows x 16
+// Partial products of R := A X for a symmetric A, or a general one when
+// `general` (block row I, blocks J in [J0, J1)): Rp[(split)*n*16 + i + col*n].  4 waves x (32 rows x 16
 // columns) of v_mfma_f64_16x16x4f64 per 128-row block, X staged through LDS
 // in fragment order, A fragments read directly (transposed for the blocks
 // taken from the other triangle).
 __global__ __launch_bounds__(256) void symm_partial_kernel(int n, int nrhs,
                                                            const double* __restrict__ A, long lda,
-                                                           int upper, const double* __restrict__ X,
+                                                           int upper, int general,
+                                                           const double* __restrict__ X,
                                                            long ldx, int jchunk,
                                                            double* __restrict__ Rp) {
   __shared__ double Xf[NB * 16];
@@ -132,7 +185,7 @@ __global__ __launch_bounds__(256) void symm_partial_kernel(int n, int nrhs,
     }
     __syncthreads();
     // direct block: stored triangle holds (I, J); mirrored: it holds (J, I)
-    const int mode = (I == J) ? 2 : ((upper ? (I < J) : (I > J)) ? 0 : 1);
+    const int mode = general ? 0 : ((I == J) ? 2 : ((upper ? (I < J) : (I > J)) ? 0 : 1));
     const bool full = (r0 + NB <= n) && (c0 + NB <= n);
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -360,7 +413,7 @@ int lp_panel(hipStream_t s, int n, LpFactor& f, double* L, long ldl, int k, int 
   const int m = n - k;
   hipLaunchKernelGGL(s2d_panel_kernel, dim3(nblocks((long)m * kb)), dim3(256), 0, s, m, kb,
                      f.As + (long)k + (long)k * n, (long)n, L + (long)k + (long)k * ldl, ldl,
-                     f.amax + slot);
+                     f.amax + slot, 1);
   FLA_HIP_CHECK(hipGetLastError());
   if (g_mp_panel_inverse)
     FLA_TRY(chol_panel_f64_inv(s, n, L, ldl, k, kb, dinfo));
@@ -370,7 +423,7 @@ int lp_panel(hipStream_t s, int n, LpFactor& f, double* L, long ldl, int k, int 
   if (m2 > 0) {
     const double* L21 = L + (long)(k + kb) + (long)k * ldl;
     FLA_TRY(launch_lp_absmax(s, m2, kb, L21, ldl, f.amax + slot));
-    FLA_TRY(launch_lp_pack(s, bf16, m2, kb, L21, ldl, f.amax + slot, f.P[slot]));
+    FLA_TRY(launch_lp_pack(s, bf16, m2, kb, L21, ldl, f.amax + slot, f.P[slot], false));
   }
   return FLA_HIP_OK;
 }
@@ -403,7 +456,7 @@ int lp_chol_lower(int n, LpFactor& f, double* L, long ldl, bool bf16, int nb, in
     float* C2 = f.As + (long)k2 + (long)k2 * n;
     void* tok = fla_stats_begin(su);
     FLA_TRY(launch_lp_update(su, bf16, true, m2, kb2, kb, f.P[slot], 0, f.P[slot], 0, -1.0f,
-                             f.amax + slot, C2, n, dinfo));
+                             f.amax + slot, f.amax + slot, C2, n, dinfo));
     fla_stats_end(su, tok, 4, lower_update_flops(m2, kb2, kb));
     hipEvent_t next_ready = next_event();
     FLA_HIP_CHECK(hipEventRecord(next_ready, su));
@@ -411,7 +464,8 @@ int lp_chol_lower(int n, LpFactor& f, double* L, long ldl, bool bf16, int nb, in
     if (m3 > 0) {
       tok = fla_stats_begin(su);
       FLA_TRY(launch_lp_update(su, bf16, true, m3, m3, kb, f.P[slot], kb2, f.P[slot], kb2, -1.0f,
-                               f.amax + slot, C2 + (long)kb2 + (long)kb2 * n, n, dinfo));
+                               f.amax + slot, f.amax + slot, C2 + (long)kb2 + (long)kb2 * n, n,
+                               dinfo));
       fla_stats_end(su, tok, 4, lower_update_flops(m3, m3, kb));
     }
     FLA_HIP_CHECK(hipStreamWaitEvent(sp, next_ready, 0));
@@ -472,15 +526,175 @@ int mp_chol(int upper, int n, const double* A, long lda, bool bf16, double* L, l
   return FLA_HIP_OK;
 }
 
+// ---- reduced-precision LU ----------------------------------------------------
+
+struct LpLu {
+  float* As = nullptr;        // FP32 trailing matrix, ld n
+  void* P[2] = {};            // packed L21 (double-buffered across steps)
+  void* Q = nullptr;          // packed U12^T of the current step (both column ranges)
+  unsigned* amax = nullptr;   // [0..1] L21 slots, [2..3] U12 ranges (a) and (b)
+  long q_elems = 0;
+};
+
+// Panel step k: F[k:n, k:k+kb] := FP64 LU with partial pivoting of the FP32
+// trailing values, then L21 packed (scaled) into P[set].
+int lp_lu_panel(hipStream_t s, int n, LpLu& f, double* F, long ldf, int* piv, int k, int kb,
+                bool bf16, int set, int* dinfo) {
+  const int m = n - k;
+  hipLaunchKernelGGL(s2d_panel_kernel, dim3(nblocks((long)m * kb)), dim3(256), 0, s, m, kb,
+                     f.As + (long)k + (long)k * n, (long)n, F + (long)k + (long)k * ldf, ldf,
+                     f.amax + set, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  FLA_TRY(fla_lu_panel_run(s, set, m, kb, F + (long)k + (long)k * ldf, ldf, k, piv, dinfo));
+  const int m2 = m - kb;
+  if (m2 > 0) {
+    const double* L21 = F + (long)(k + kb) + (long)k * ldf;
+    FLA_TRY(launch_lp_absmax(s, m2, kb, L21, ldf, f.amax + set));
+    FLA_TRY(launch_lp_pack(s, bf16, m2, kb, L21, ldf, f.amax + set, f.P[set], false));
+  }
+  return FLA_HIP_OK;
+}
+
+// Columns [c0, c0+nc) after the panel at (k, kb): the panel's row
+// interchanges on the FP32 columns, U12 := inv(L11) * A12 in FP64 (into F),
+// then A22 -= L21 * U12 in reduced precision.  `range` selects U12's absmax
+// slot and where its packed rows start in Q.
+int lp_lu_cols(hipStream_t s, int n, LpLu& f, double* F, long ldf, int k, int kb, int c0, int nc,
+               int qrow0, int range, bool bf16, int set, const int* dinfo) {
+  if (nc <= 0) return FLA_HIP_OK;
+  FLA_TRY(fla_lu_gather_rows_f(s, set, nc, f.As + (long)k + (long)c0 * n, n));
+  unsigned* aq = f.amax + 2 + range;
+  double* U12 = F + (long)k + (long)c0 * ldf;
+  hipLaunchKernelGGL(s2d_panel_kernel, dim3(nblocks((long)kb * nc)), dim3(256), 0, s, kb, nc,
+                     f.As + (long)k + (long)c0 * n, (long)n, U12, ldf, aq, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  FLA_TRY(launch_trsm_llnu(s, nc, kb, F + (long)k + (long)k * ldf, ldf, U12, ldf));
+  const int m2 = n - k - kb;
+  if (m2 <= 0) return FLA_HIP_OK;
+  FLA_TRY(launch_lp_absmax(s, kb, nc, U12, ldf, aq));
+  // fragment-order rows of Q start at row qrow0 (a multiple of 16): 1 KB per
+  // (16-row tile, k-block)
+  const int kbw = bf16 ? 32 : 16;
+  const long nkb = (kb + kbw - 1) / kbw;
+  char* Qr = static_cast<char*>(f.Q) + (long)(qrow0 / 16) * nkb * 1024;
+  FLA_TRY(launch_lp_pack(s, bf16, nc, kb, U12, ldf, aq, Qr, true));
+  void* tok = fla_stats_begin(s);
+  FLA_TRY(launch_lp_update(s, bf16, false, m2, nc, kb, f.P[set], 0, Qr, 0, -1.0f, f.amax + set,
+                           aq, f.As + (long)(k + kb) + (long)c0 * n, n, dinfo));
+  fla_stats_end(s, tok, 5, 2.0 * m2 * (double)nc * kb);
+  return FLA_HIP_OK;
+}
+
+// Right-looking blocked LU with look-ahead, the schedule of fla_hip_dgetrf:
+// the next panel's columns are updated first, then that panel (s_panel)
+// overlaps the update of the remaining columns (s_update_lp).
+int lp_lu(int n, LpLu& f, double* F, long ldf, int* piv, bool bf16, int nb, int* dinfo) {
+  FlaHipCtx& c = fla_ctx();
+  hipStream_t sp = c.s_panel, su = c.s_update_lp;
+  int evi = 0;
+  auto next_event = [&]() {
+    hipEvent_t e = c.ev[evi];
+    evi = (evi + 1) % FlaHipCtx::kEvents;
+    return e;
+  };
+  hipEvent_t start = next_event();
+  FLA_HIP_CHECK(hipEventRecord(start, c.s_update));
+  FLA_HIP_CHECK(hipStreamWaitEvent(sp, start, 0));
+  FLA_HIP_CHECK(hipStreamWaitEvent(su, start, 0));
+  int kb = n < nb ? n : nb;
+  FLA_TRY(lp_lu_panel(sp, n, f, F, ldf, piv, 0, kb, bf16, 0, dinfo));
+  for (int k = 0, step = 0; k < n; ++step) {
+    const int set = step & 1;
+    hipEvent_t pd = next_event();
+    FLA_HIP_CHECK(hipEventRecord(pd, sp));
+    FLA_HIP_CHECK(hipStreamWaitEvent(su, pd, 0));
+    // interchanges on the FP64 columns left of the panel
+    FLA_TRY(fla_lu_gather_rows(su, set, k, F + (long)k, ldf));
+    const int k2 = k + kb;
+    const int kb2 = (n - k2) < nb ? (n - k2) : nb;
+    FLA_TRY(lp_lu_cols(su, n, f, F, ldf, k, kb, k2, kb2 > 0 ? kb2 : 0, 0, 0, bf16, set, dinfo));
+    hipEvent_t ready = next_event();
+    FLA_HIP_CHECK(hipEventRecord(ready, su));
+    FLA_TRY(lp_lu_cols(su, n, f, F, ldf, k, kb, k2 + kb2, n - k2 - kb2, kb2, 1, bf16, set,
+                       dinfo));
+    if (kb2 <= 0) break;
+    FLA_HIP_CHECK(hipStreamWaitEvent(sp, ready, 0));
+    FLA_TRY(lp_lu_panel(sp, n, f, F, ldf, piv, k2, kb2, bf16, set ^ 1, dinfo));
+    k = k2;
+    kb = kb2;
+  }
+  hipEvent_t fin = next_event();
+  FLA_HIP_CHECK(hipEventRecord(fin, sp));
+  FLA_HIP_CHECK(hipStreamWaitEvent(c.s_update, fin, 0));
+  hipEvent_t fin_u = next_event();
+  FLA_HIP_CHECK(hipEventRecord(fin_u, su));
+  FLA_HIP_CHECK(hipStreamWaitEvent(c.s_update, fin_u, 0));
+  return FLA_HIP_OK;
+}
+
+// Reduced-precision LU of the general n x n A: FP64-valued factors into F
+// (unit lower L below the diagonal, U on and above), FLAME-native pivots into
+// piv.  *info: kSuccess or the 0-based column of the first null pivot;
+// *ovf = 1 when A does not fit in FP32 (nothing factored then).
+int mp_lu(int n, const double* A, long lda, bool bf16, double* F, long ldf, int* piv, int* info,
+          int* ovf) {
+  FlaHipCtx& c = fla_ctx();
+  hipStream_t s = c.s_update;
+  const int nb = 128;
+  const size_t as_bytes = ((size_t)n * n * sizeof(float) + 255) & ~(size_t)255;
+  const size_t p_bytes = ((size_t)lp_panel_elems(n, nb, bf16) * (bf16 ? 2 : 4) + 255) &
+                         ~(size_t)255;
+  const size_t q_bytes = ((size_t)lp_panel_elems(n + nb, nb, bf16) * (bf16 ? 2 : 4) + 255) &
+                         ~(size_t)255;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace_slot(1, as_bytes + 2 * p_bytes + q_bytes + 256, &w));
+  LpLu f;
+  char* base = static_cast<char*>(w);
+  f.As = reinterpret_cast<float*>(base);
+  f.P[0] = base + as_bytes;
+  f.P[1] = base + as_bytes + p_bytes;
+  f.Q = base + as_bytes + 2 * p_bytes;
+  f.amax = reinterpret_cast<unsigned*>(base + as_bytes + 2 * p_bytes + q_bytes);
+  int* dinfo = c.dscratch;      // [0] info
+  int* derr = c.dscratch + 4;   // panel wait-failure flag
+  int* dovf = c.dscratch + 12;  // overflow flag
+  FLA_TRY(fla_lu_panel_setup(n, derr));
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, dinfo, (int)kSuccess);
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, derr, 0);
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, dovf, 0);
+  FLA_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(ge2s_kernel, dim3(nblocks((long)n * n)), dim3(256), 0, s, n, n, A, lda,
+                     f.As, (long)n, dovf);
+  FLA_HIP_CHECK(hipGetLastError());
+  int h[2] = {0, 0};
+  FLA_HIP_CHECK(hipMemcpyAsync(h, dovf, sizeof(int), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  *ovf = h[0];
+  *info = kSuccess;
+  if (h[0]) return FLA_HIP_OK;
+  FLA_TRY(lp_lu(n, f, F, ldf, piv, bf16, nb, dinfo));
+  FLA_HIP_CHECK(hipMemcpyAsync(&h[0], dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipMemcpyAsync(&h[1], derr, sizeof(int), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  fla_stats_harvest();
+  if (h[1] != 0) return FLA_HIP_ESPIN;
+  *info = h[0];
+  return FLA_HIP_OK;
+}
+
 // ---- refinement ----------------------------------------------------------------
 
 struct MpState {
   hipStream_t s = nullptr;
   int n = 0, nrhs = 0, upper = 0;
+  bool general = false;  // LU (general A) instead of Cholesky
   const double* A = nullptr;
   long lda = 0;
-  double* L = nullptr;   // FP64-valued reduced-precision factor (lower), ld n
-  double* W = nullptr;   // inverted 128-blocks of L
+  double* L = nullptr;   // FP64-valued reduced-precision factor (lower, or LU), ld n
+  double* W = nullptr;   // inverted 128-blocks of L (unit lower for LU)
+  double* W2 = nullptr;  // inverted 128-blocks of U (LU)
+  int* perm = nullptr;   // LU: row q of P A is row perm[q] of A
+  double* T = nullptr;   // LU: n x nrhs staging block
   int* flags = nullptr;  // solve counters
   double* Rp = nullptr;  // symm partials
   int nsplit = 1, jchunk = 1;
@@ -488,8 +702,22 @@ struct MpState {
   int solves = 0, matvecs = 0;
 };
 
-// X := (L L^T)^-1 X on an n x nrhs block.
+// X := (L L^T)^-1 X, or X := (P^T L U)^-1 X = U^-1 L^-1 P X, on an n x nrhs
+// block.
 int mp_apply_prec(MpState& st, double* X, long ldx) {
+  if (st.general) {
+    hipLaunchKernelGGL(perm_rows_kernel, dim3(nblocks((long)st.n * st.nrhs)), dim3(256), 0, st.s,
+                       st.n, st.nrhs, st.perm, X, ldx, st.T, (long)st.n);
+    FLA_HIP_CHECK(hipGetLastError());
+    FLA_TRY(launch_trsm_left_narrow(st.s, true, false, st.n, st.nrhs, st.L, st.n, st.W, st.T,
+                                    st.n, st.flags, st.flags + 1));
+    FLA_TRY(launch_trsm_left_narrow(st.s, false, false, st.n, st.nrhs, st.L, st.n, st.W2, st.T,
+                                    st.n, st.flags, st.flags + 1));
+    FLA_HIP_CHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, st.T, sizeof(double) * st.n,
+                                   sizeof(double) * st.n, st.nrhs, hipMemcpyDeviceToDevice, st.s));
+    ++st.solves;
+    return FLA_HIP_OK;
+  }
   FLA_TRY(launch_trsm_left_narrow(st.s, true, false, st.n, st.nrhs, st.L, st.n, st.W, X, ldx,
                                   st.flags, st.flags + 1));
   FLA_TRY(launch_trsm_left_narrow(st.s, false, true, st.n, st.nrhs, st.L, st.n, st.W, X, ldx,
@@ -498,14 +726,15 @@ int mp_apply_prec(MpState& st, double* X, long ldx) {
   return FLA_HIP_OK;
 }
 
-// R := B - A X, or R := A X when B is null (FP64, symmetric A).
+// R := B - A X, or R := A X when B is null (FP64, symmetric or general A).
 int mp_residual(MpState& st, const double* B, long ldb, const double* X, long ldx, double* R,
                 long ldr) {
   const int nb = (st.n + NB - 1) / NB;
   for (int c = 0; c < st.nrhs; c += 16) {
     const int nr = (st.nrhs - c) < 16 ? (st.nrhs - c) : 16;
     hipLaunchKernelGGL(symm_partial_kernel, dim3(nb, st.nsplit), dim3(256), 0, st.s, st.n, nr,
-                       st.A, st.lda, st.upper, X + (long)c * ldx, ldx, st.jchunk, st.Rp);
+                       st.A, st.lda, st.upper, st.general ? 1 : 0, X + (long)c * ldx, ldx,
+                       st.jchunk, st.Rp);
     FLA_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(symm_finish_kernel, dim3(nblocks((long)st.n * nr)), dim3(256), 0, st.s,
                        st.n, nr, B ? B + (long)c * ldb : nullptr, ldb, st.Rp, st.nsplit,
@@ -680,9 +909,30 @@ extern "C" int fla_hip_mp_chol(int uplo, int n, const double* A, int lda, int pr
   return FLA_HIP_OK;
 }
 
-extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* B,
-                              int ldb, double* X, int ldx, const fla_hip_mp_opts* opts, int* iter,
-                              int* info) {
+// Host-side walk of FLAME-native pivots into a row permutation: row q of
+// P A is row perm[q] of A.
+static int pivots_to_perm(hipStream_t s, int n, const int* dpiv, int* dperm) {
+  std::vector<int> p(n), perm(n);
+  FLA_HIP_CHECK(hipMemcpyAsync(p.data(), dpiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int i = 0; i < n; ++i) {
+    const int t = i + p[i];
+    if (t < i || t >= n) return FLA_HIP_EARG;
+    const int a = perm[i];
+    perm[i] = perm[t];
+    perm[t] = a;
+  }
+  FLA_HIP_CHECK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  return FLA_HIP_OK;
+}
+
+// dsposv_ (general = false) and dsgesv_ (general = true): reduced-precision
+// factor, refinement with the FP64 residual, FP64 fall-back.
+static int mp_solve(bool general, int uplo, int n, int nrhs, double* A, int lda, int* piv,
+                    const double* B, int ldb, double* X, int ldx, const fla_hip_mp_opts* opts,
+                    int* iter, int* info) {
   FLA_TRY(fla_ensure_ready());
   fla_hip_mp_opts o;
   o.precision = FLA_MP_FP32;
@@ -691,12 +941,13 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   o.gmres_restart = 20;
   o.gmres_tol = 1e-4;
   if (opts) o = *opts;
-  if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
+  if (!general && uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
   if (o.precision != FLA_MP_FP32 && o.precision != FLA_MP_BF16) return FLA_HIP_EARG;
   if (o.refinement != FLA_IR_CLASSIC && o.refinement != FLA_IR_GMRES) return FLA_HIP_EARG;
   if (n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1) ||
       ldx < (n > 1 ? n : 1) || o.max_iter < 0 || o.gmres_restart < 1)
     return FLA_HIP_EARG;
+  if (general && n > 0 && !piv) return FLA_HIP_EARG;
   *iter = 0;
   *info = 0;
   for (double& v : g_mp_stats) v = 0.0;
@@ -713,7 +964,8 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   st.s = s;
   st.n = n;
   st.nrhs = nrhs;
-  st.upper = (uplo == kUpper);
+  st.upper = (!general && uplo == kUpper);
+  st.general = general;
   st.A = A;
   st.lda = lda;
   const int nb = (n + NB - 1) / NB;
@@ -721,20 +973,26 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   st.jchunk = (nb + st.nsplit - 1) / st.nsplit;
   const int gm = o.refinement == FLA_IR_GMRES ? o.gmres_restart : 0;
   const long blk = (long)n * nrhs;
-  // slot 2: L | W | Rp | Xw R D Wv | V (gm+1 blocks) | hbuf | flags
+  // slot 2: L | W | W2 | Rp | R D Wv Bd T | V (gm+1 blocks) | hbuf | perm | flags
   const size_t szL = (size_t)n * n, szW = (size_t)nb * NB * NB, szRp = (size_t)st.nsplit * n * 16;
   const int T64 = (n + 63) / 64;
+  const int GS = 64;  // column splits of the general norm
   size_t szV = (size_t)(gm + 1) * blk;
   if (szV < (size_t)2 * T64 * n) szV = (size_t)2 * T64 * n;  // norm partials share it
+  if (szV < (size_t)GS * n) szV = (size_t)GS * n;
   const size_t szH = (size_t)4 * (gm + 2) * nrhs + 64;
+  const size_t szW2 = general ? szW : 0;
+  const size_t szPerm = ((size_t)n + 1) / 2 + 1;  // ints, in doubles
   void* w = nullptr;
-  FLA_TRY(fla_workspace_slot(2, sizeof(double) * (szL + szW + szRp + 4 * blk + szV + szH) + 256,
-                             &w));
+  FLA_TRY(fla_workspace_slot(
+      2, sizeof(double) * (szL + szW + szW2 + szRp + 5 * blk + szV + szH + szPerm) + 256, &w));
   double* p = static_cast<double*>(w);
   st.L = p;
   p += szL;
   st.W = p;
   p += szW;
+  st.W2 = p;
+  p += szW2;
   st.Rp = p;
   p += szRp;
   double* R = p;
@@ -745,21 +1003,33 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   p += blk;
   double* Bd = p;  // dense copy of B (ld n)
   p += blk;
+  st.T = p;
+  p += blk;
   double* V = p;
   p += szV;
   st.hbuf = p;
   p += szH;
+  st.perm = reinterpret_cast<int*>(p);
+  p += szPerm;
   st.flags = reinterpret_cast<int*>(p);
 
   FLA_HIP_CHECK(hipMemcpy2DAsync(Bd, sizeof(double) * n, B, sizeof(double) * ldb,
                                  sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
-  // ||A||_inf and cte (dsposv.c:295-297)
-  double* Prow = V;  // the Krylov space is free before refinement
-  double* Pcol = Prow + (long)T64 * n;
-  hipLaunchKernelGGL(syminf_partial_kernel, dim3((unsigned)((long)T64 * (T64 + 1) / 2)),
-                     dim3(256), 0, s, n, A, (long)lda, st.upper, Prow, Pcol);
-  hipLaunchKernelGGL(syminf_rows_kernel, dim3(nblocks(n)), dim3(256), 0, s, n, Prow, Pcol);
-  hipLaunchKernelGGL(vmax_kernel, dim3(1), dim3(256), 0, s, n, Prow, st.hbuf);
+  // ||A||_inf and cte (dsposv.c:295-297, dsgesv.c:291-293)
+  if (general) {
+    const int jchunk = (n + GS - 1) / GS;
+    hipLaunchKernelGGL(geinf_partial_kernel, dim3((n + 255) / 256, GS), dim3(256), 0, s, n, A,
+                       (long)lda, jchunk, V);
+    hipLaunchKernelGGL(geinf_rows_kernel, dim3(nblocks(n)), dim3(256), 0, s, n, GS, V);
+    hipLaunchKernelGGL(vmax_kernel, dim3(1), dim3(256), 0, s, n, V, st.hbuf);
+  } else {
+    double* Prow = V;  // the Krylov space is free before refinement
+    double* Pcol = Prow + (long)T64 * n;
+    hipLaunchKernelGGL(syminf_partial_kernel, dim3((unsigned)((long)T64 * (T64 + 1) / 2)),
+                       dim3(256), 0, s, n, A, (long)lda, st.upper, Prow, Pcol);
+    hipLaunchKernelGGL(syminf_rows_kernel, dim3(nblocks(n)), dim3(256), 0, s, n, Prow, Pcol);
+    hipLaunchKernelGGL(vmax_kernel, dim3(1), dim3(256), 0, s, n, Prow, st.hbuf);
+  }
   FLA_HIP_CHECK(hipGetLastError());
   double anrm = 0.0;
   FLA_HIP_CHECK(hipMemcpyAsync(&anrm, st.hbuf, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -777,7 +1047,11 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   int finfo = kSuccess;
   if (code == 0) {
     int ovf = 0;
-    FLA_TRY(mp_chol(st.upper, n, A, lda, o.precision == FLA_MP_BF16, st.L, n, &finfo, &ovf));
+    const bool bf16 = o.precision == FLA_MP_BF16;
+    if (general)
+      FLA_TRY(mp_lu(n, A, lda, bf16, st.L, n, piv, &finfo, &ovf));
+    else
+      FLA_TRY(mp_chol(st.upper, n, A, lda, bf16, st.L, n, &finfo, &ovf));
     if (ovf)
       code = -2;
     else if (finfo != kSuccess)
@@ -786,7 +1060,13 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   FLA_HIP_CHECK(hipEventRecord(e1, s));
   int it = 0;
   if (code == 0) {
-    FLA_TRY(launch_trtri_blocks(s, n, st.L, n, false, false, st.W));
+    if (general) {
+      FLA_TRY(pivots_to_perm(s, n, piv, st.perm));
+      FLA_TRY(launch_trtri_blocks(s, n, st.L, n, false, true, st.W));
+      FLA_TRY(launch_trtri_blocks(s, n, st.L, n, true, false, st.W2));
+    } else {
+      FLA_TRY(launch_trtri_blocks(s, n, st.L, n, false, false, st.W));
+    }
     // x = M^-1 b, r = b - A x
     FLA_HIP_CHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, Bd, sizeof(double) * n,
                                    sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
@@ -837,6 +1117,23 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   FLA_HIP_CHECK(hipEventRecord(e2, s));
   if (code == 0) {
     *iter = it;
+  } else if (general) {
+    // FP64 factorization and solve (dsgesv.c:397-402): A and piv overwritten.
+    *iter = code;
+    int dinfo = kSuccess;
+    FLA_TRY(fla_hip_dgetrf(n, n, A, lda, piv, &dinfo));
+    if (dinfo != kSuccess) {
+      *info = dinfo + 1;
+    } else {
+      FLA_HIP_CHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, Bd, sizeof(double) * n,
+                                     sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, s));
+      FLA_TRY(pivots_to_perm(s, n, piv, st.perm));
+      hipLaunchKernelGGL(perm_rows_kernel, dim3(nblocks(blk)), dim3(256), 0, s, n, nrhs, st.perm,
+                         Bd, (long)n, X, (long)ldx);
+      FLA_HIP_CHECK(hipGetLastError());
+      FLA_TRY(fla_hip_dtrsm(kLeft, kLower, kNoTrans, kUnit, n, nrhs, 1.0, A, lda, X, ldx));
+      FLA_TRY(fla_hip_dtrsm(kLeft, kUpper, kNoTrans, kNonUnit, n, nrhs, 1.0, A, lda, X, ldx));
+    }
   } else {
     // FP64 factorization and solve (dsposv.c:403-405): A is overwritten.
     *iter = code;
@@ -865,6 +1162,31 @@ extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, con
   FLA_HIP_CHECK(hipEventDestroy(e0));
   FLA_HIP_CHECK(hipEventDestroy(e1));
   FLA_HIP_CHECK(hipEventDestroy(e2));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* B,
+                              int ldb, double* X, int ldx, const fla_hip_mp_opts* opts, int* iter,
+                              int* info) {
+  return mp_solve(false, uplo, n, nrhs, A, lda, nullptr, B, ldb, X, ldx, opts, iter, info);
+}
+
+extern "C" int fla_hip_dsgesv(int n, int nrhs, double* A, int lda, int* p, const double* B,
+                              int ldb, double* X, int ldx, const fla_hip_mp_opts* opts, int* iter,
+                              int* info) {
+  return mp_solve(true, kLower, n, nrhs, A, lda, p, B, ldb, X, ldx, opts, iter, info);
+}
+
+extern "C" int fla_hip_mp_lu(int n, const double* A, int lda, int precision, double* F, int ldf,
+                             int* p, int* info) {
+  FLA_TRY(fla_ensure_ready());
+  if (precision != FLA_MP_FP32 && precision != FLA_MP_BF16) return FLA_HIP_EARG;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldf < (n > 1 ? n : 1)) return FLA_HIP_EARG;
+  *info = kSuccess;
+  if (n == 0) return FLA_HIP_OK;
+  int ovf = 0;
+  FLA_TRY(mp_lu(n, A, lda, precision == FLA_MP_BF16, F, ldf, p, info, &ovf));
+  if (ovf) return FLA_HIP_ERANGE;
   return FLA_HIP_OK;
 }
 

@@ -218,6 +218,31 @@ __global__ void refl_z_kernel(int n, const double* __restrict__ v, const double*
   z[i] = tau * w[i] - 0.5 * tau * tau * d[0] * v[i];
 }
 
+// A -= tau x y^T with tau = 2 / d[1] (d[1] = v.v of the reflector).
+__global__ void rank1_kernel(int n, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ d, double* __restrict__ A, long lda) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  A[(long)i + (long)j * lda] -= (2.0 / d[1]) * x[i] * y[j];
+}
+
+// w[j] = sum_i A(i, j) v[i] (one workgroup per column, fixed order).
+__global__ void gemv_t_kernel(int n, const double* __restrict__ A, long lda,
+                              const double* __restrict__ v, double* __restrict__ w) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += A[(long)i + (long)j * lda] * v[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) w[j] = red[0];
+}
+
 // A -= v z^T + z v^T
 __global__ void rank2_kernel(int n, const double* __restrict__ v, const double* __restrict__ z,
                              double* __restrict__ A, long lda) {
@@ -358,6 +383,43 @@ extern "C" int fla_hip_gen_spd_reflectors(unsigned long long seed, int n, int nr
     hipLaunchKernelGGL(refl_z_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, v, wv, d, z);
     hipLaunchKernelGGL(rank2_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, n, v, z,
                        A, (long)lda);
+    FLA_HIP_CHECK(hipGetLastError());
+  }
+  return FLA_HIP_OK;
+}
+
+// General A = Q1 diag(s) Q2^T: A := diag(s), then for each reflector pair
+// A := H_r A (A -= tau v (A^T v)^T) and A := A G_r (A -= tau (A u) u^T), with
+// v from seed + r + 1 and u from seed2 + r + 1 (fla_hip_gen_normal).
+extern "C" int fla_hip_gen_ge_reflectors(unsigned long long seed, unsigned long long seed2, int n,
+                                         int nrefl, double kappa, double* A, int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (n <= 0) return FLA_HIP_OK;
+  if (nrefl < 0 || lda < n || !(kappa >= 1.0)) return FLA_HIP_EARG;
+  hipStream_t s = fla_ctx().s_update;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace_slot(3, sizeof(double) * (2 * (size_t)n + 8), &w));
+  double* v = static_cast<double*>(w);
+  double* wv = v + n;
+  double* d = wv + n;
+  const long nn = (long)n * n;
+  const unsigned gnn = (unsigned)((nn + 255) / 256);
+  hipLaunchKernelGGL(logdiag_kernel, dim3(gnn), dim3(256), 0, s, n, kappa, A, (long)lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  for (int r = 0; r < nrefl; ++r) {
+    // left: A := (I - tau v v^T) A
+    hipLaunchKernelGGL(gen_kernel, dim3((n + 255) / 256), dim3(256), 0, s, seed + r + 1, n, 1, v,
+                       (long)n, 1);
+    hipLaunchKernelGGL(gemv_t_kernel, dim3(n), dim3(256), 0, s, n, A, (long)lda, v, wv);
+    hipLaunchKernelGGL(dot2_kernel, dim3(1), dim3(256), 0, s, n, v, v, d);
+    hipLaunchKernelGGL(rank1_kernel, dim3(gnn), dim3(256), 0, s, n, v, wv, d, A, (long)lda);
+    // right: A := A (I - tau u u^T)
+    hipLaunchKernelGGL(gen_kernel, dim3((n + 255) / 256), dim3(256), 0, s, seed2 + r + 1, n, 1, v,
+                       (long)n, 1);
+    FLA_TRY(launch_dgemm_mfma(s, false, false, 0, n, 1, n, 1.0, A, lda, v, n, 0.0, wv, n,
+                              nullptr));
+    hipLaunchKernelGGL(dot2_kernel, dim3(1), dim3(256), 0, s, n, v, v, d);
+    hipLaunchKernelGGL(rank1_kernel, dim3(gnn), dim3(256), 0, s, n, wv, v, d, A, (long)lda);
     FLA_HIP_CHECK(hipGetLastError());
   }
   return FLA_HIP_OK;

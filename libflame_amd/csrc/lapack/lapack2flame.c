@@ -266,3 +266,60 @@ int dsposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* 
   }
   return 0;
 }
+
+/* dsgesv_ (src/map/common/lapacksrc/f2c/dsgesv.c:189): reduced-precision LU
+   with partial pivoting (FP32 trailing updates) + refinement with an FP64
+   residual, the netlib stopping rule, 30-step limit and FP64 fall-back (A
+   then holds the FP64 factors).  ipiv returns LAPACK (1-based absolute)
+   pivots of the factorization used.  work / swork are accepted for the ABI
+   and not used: the workspace lives on the device. */
+int dsgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb, double* x,
+            int* ldx, double* work, float* swork, int* iter, int* info) {
+  int st, i;
+  double *dA = NULL, *dB = NULL, *dX = NULL;
+  int* dp = NULL;
+  (void)work;
+  (void)swork;
+  *info = 0;
+  *iter = 0;
+  if (*n < 0)
+    *info = -1;
+  else if (*nrhs < 0)
+    *info = -2;
+  else if (*lda < imax(1, *n))
+    *info = -4;
+  else if (*ldb < imax(1, *n))
+    *info = -7;
+  else if (*ldx < imax(1, *n))
+    *info = -9;
+  if (*info != 0) {
+    xerbla("DSGESV", -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)*n * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dB, sizeof(double) * (size_t)*n * (size_t)imax(1, *nrhs));
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dX, sizeof(double) * (size_t)*n * (size_t)imax(1, *nrhs));
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dp, sizeof(int) * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *n, a, *lda, dA, *n);
+  if (st == FLA_HIP_OK && *nrhs > 0) st = fla_hip_dmatrix_h2d(*n, *nrhs, b, *ldb, dB, *n);
+  if (st == FLA_HIP_OK)
+    st = fla_hip_dsgesv(*n, *nrhs, dA, *n, dp, dB, *n, dX, *n, NULL, iter, info);
+  if (st == FLA_HIP_OK && *nrhs > 0) st = fla_hip_dmatrix_d2h(*n, *nrhs, dX, *n, x, *ldx);
+  if (st == FLA_HIP_OK && *iter < 0) st = fla_hip_dmatrix_d2h(*n, *n, dA, *n, a, *lda);
+  if (st == FLA_HIP_OK && *nrhs > 0) {
+    /* FLAME-native (0-based relative) -> LAPACK (1-based absolute) */
+    st = fla_hip_memcpy_d2h(ipiv, dp, sizeof(int) * (size_t)*n);
+    if (st == FLA_HIP_OK)
+      for (i = 0; i < *n; ++i) ipiv[i] += i + 1;
+  }
+  fla_hip_free(dA);
+  fla_hip_free(dB);
+  fla_hip_free(dX);
+  fla_hip_free(dp);
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: dsgesv_ failed: %s\n", fla_hip_error_string(st));
+    *info = -100;
+  }
+  return 0;
+}
