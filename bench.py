@@ -16,6 +16,7 @@ import argparse
 import json
 import os
 import sys
+import ctypes as C
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -171,10 +172,78 @@ def bench_mixed(fla, np, n, nrhs=8):
     return res
 
 
-def backward_error(fla, np, A0, B, X, R, n, nrhs, anorm):
+def bench_mixed_lu(fla, np, n, nrhs=8):
+    """Mixed-precision general solve (fla_hip_dsgesv) against the plain FP64
+    solve (FLA_LU_piv, the row interchanges and two triangular solves) on
+    A = Q1 diag(s) Q2^T with independent products of 64 reflectors each,
+    s log-spaced on [1, kappa]; 8 N(0,1) RHS."""
+    lib = fla.lib()
+    res = {"n": n, "nrhs": nrhs, "cases": []}
+    A0 = fla.DeviceMatrix(n, n)
+    A = fla.DeviceMatrix(n, n)
+    B = fla.DeviceMatrix(n, nrhs)
+    X = fla.DeviceMatrix(n, nrhs)
+    X64 = fla.DeviceMatrix(n, nrhs)
+    R = fla.DeviceMatrix(n, nrhs)
+    p = fla.DeviceMatrix(n, 1, np.int32)
+    fla.check(lib.fla_hip_gen_normal(5, n, nrhs, B.ptr, n))
+    for kappa in (1e2, 1e4, 1e6):
+        log(f"mixed LU: kappa={kappa:g}, n={n}")
+        fla.check(lib.fla_hip_gen_ge_reflectors(11, 1011, n, 64, kappa, A0.ptr, n))
+        fla.synchronize()
+        t64 = []
+        for rep in range(2):
+            copy(fla, A, A0)
+            copy(fla, X64, B)
+            fla.synchronize()
+            t0 = time.perf_counter()
+            info = C.c_int()
+            fla.check(lib.fla_hip_dgetrf(n, n, A.ptr, n, p.ptr, C.byref(info)))
+            fla.check(lib.fla_hip_dlaswp(nrhs, X64.ptr, n, 0, n - 1, p.ptr, 1))
+            fla.check(lib.fla_hip_dtrsm(fla.FLA_LEFT, fla.FLA_LOWER_TRIANGULAR,
+                                        fla.FLA_NO_TRANSPOSE, fla.FLA_UNIT_DIAG, n, nrhs, 1.0,
+                                        A.ptr, n, X64.ptr, n))
+            fla.check(lib.fla_hip_dtrsm(fla.FLA_LEFT, fla.FLA_UPPER_TRIANGULAR,
+                                        fla.FLA_NO_TRANSPOSE, fla.FLA_NONUNIT_DIAG, n, nrhs, 1.0,
+                                        A.ptr, n, X64.ptr, n))
+            fla.synchronize()
+            t64.append(time.perf_counter() - t0)
+        ms64 = min(t64) * 1e3
+        anorm = _sym_inf_norm(fla, np, A0, n)
+        bwd64 = backward_error(fla, np, A0, B, X64, R, n, nrhs, anorm, general=True)
+        case = {"kappa": kappa, "fp64_ms": round(ms64, 2), "fp64_bwd_err_over_eps": bwd64}
+        for name, prec, ref in (("fp32_classic", fla.FLA_MP_FP32, fla.FLA_IR_CLASSIC),
+                                ("fp32_gmres", fla.FLA_MP_FP32, fla.FLA_IR_GMRES),
+                                ("bf16_gmres", fla.FLA_MP_BF16, fla.FLA_IR_GMRES)):
+            ts = []
+            for rep in range(2):
+                copy(fla, A, A0)
+                fla.synchronize()
+                t0 = time.perf_counter()
+                it, info, st = fla.dsgesv(A, p, B, X, precision=prec, refinement=ref)
+                fla.synchronize()
+                ts.append(time.perf_counter() - t0)
+            bwd = backward_error(fla, np, A0, B, X, R, n, nrhs, anorm, general=True)
+            xh, x64 = X.download(), X64.download()
+            fwd = float(np.max(np.abs(xh - x64)) / np.max(np.abs(x64)))
+            case[name] = {"ms": round(min(ts) * 1e3, 2), "iter": it, "info": info,
+                          "bwd_err_over_eps": bwd, "fwd_err_vs_fp64": fwd,
+                          "factor_ms": round(st["factor_ms"], 2),
+                          "refine_ms": round(st["refine_ms"], 2),
+                          "prec_solves": int(st["prec_solves"]),
+                          "gmres_iters": int(st["gmres_iters"]),
+                          "time_ratio_vs_fp64": round(min(ts) * 1e3 / ms64, 3)}
+            log(f"  {name}: {case[name]}")
+        res["cases"].append(case)
+    del A0, A, B, X, X64, R, p
+    return res
+
+
+def backward_error(fla, np, A0, B, X, R, n, nrhs, anorm, general=False):
     """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) / eps, with the
-    product computed in FP64 on the GPU from the full symmetric A."""
-    fla.check(fla.lib().fla_hip_dmatmul_check(1, fla.FLA_LOWER_TRIANGULAR, fla.FLA_NO_TRANSPOSE,
+    product computed in FP64 on the GPU from the full (symmetric or general) A."""
+    fla.check(fla.lib().fla_hip_dmatmul_check(0 if general else 1, fla.FLA_LOWER_TRIANGULAR,
+                                               fla.FLA_NO_TRANSPOSE,
                                                fla.FLA_NONUNIT_DIAG, n, nrhs, A0.ptr, n, X.ptr,
                                                n, 0.0, R.ptr, n))
     fla.synchronize()
@@ -425,6 +494,7 @@ def main():
 
     if not args.no_mixed:
         extras["mixed"] = bench_mixed(fla, np, args.mp_n)
+        extras["mixed_lu"] = bench_mixed_lu(fla, np, args.mp_n)
     if not args.no_batched:
         extras["batched"] = bench_batched(fla, np, args.batch_count)
     if not args.no_adjoint:

@@ -4,6 +4,8 @@
 // the general row-interchange kernel behind FLA_Apply_pivots / dlaswp_.
 #include "fla_hip_internal.h"
 
+#include <vector>
+
 namespace {
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
@@ -55,39 +57,8 @@ __global__ void shift_pivots_kernel(int to_lapack, int m, int* p) {
   if (i < m) p[i] += to_lapack ? (i + 1) : -(i + 1);
 }
 
-// Row order produced by the interchanges i <-> i + p[i*incp] for i in
-// [k1, k2] (ascending for incp > 0, descending for incp < 0): perm[q] is the
-// source row of final row q.  The interchanges form a chain, so one lane walks
-// them in order; the identity part is filled by the whole grid first.
-__global__ void perm_init_kernel(int m, int* perm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) perm[i] = i;
-}
-
-__global__ void perm_walk_kernel(int rows, int k1, int k2, const int* __restrict__ p, int incp,
-                                 int* perm, int* bad) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int step = incp > 0 ? 1 : -1;
-  const int pstride = incp > 0 ? incp : -incp;
-  const int ib = incp > 0 ? k1 : k2;
-  const int ie = incp > 0 ? k2 + 1 : k1 - 1;
-  for (int i = ib; i != ie; i += step) {
-    if (i < 0 || i >= rows) {
-      *bad = 1;
-      return;
-    }
-    const int t = i + p[(long)i * pstride];
-    if (t < 0 || t >= rows) {  // pivot outside the matrix: leave the rest alone
-      *bad = 1;
-      return;
-    }
-    const int a = perm[i];
-    perm[i] = perm[t];
-    perm[t] = a;
-  }
-}
-
-// new(q, j) := old(perm[q], j) for rows q in [0, rows), through a copy.
+// dst(q, :) := src(perm[q], :), perm[q] being the source row of final row q
+// of the interchanges i <-> i + p[i*incp] (fla_hip_dlaswp).
 __global__ void gather_rows_kernel(int rows, int ncols, const int* __restrict__ perm,
                                    const double* __restrict__ src, long lds,
                                    double* __restrict__ dst, long ldd) {
@@ -334,23 +305,42 @@ extern "C" int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, con
   // The interchanges touch rows up to max(i + p[i]); the permutation is built
   // over rows [0, lda) (every row index must lie inside the leading dimension).
   const int rows = lda;
+  // The interchange sequence is inherently sequential: walk it on the host
+  // (one small copy each way) rather than in a single GPU thread.
+  const int pstride = incp > 0 ? incp : -incp;
+  const long plen = (long)(k2 - k1) * pstride + 1;
+  std::vector<int> hp((size_t)plen), perm((size_t)rows);
+  FLA_HIP_CHECK(hipMemcpyAsync(hp.data(), p + (long)k1 * pstride, sizeof(int) * plen,
+                               hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  for (int q = 0; q < rows; ++q) perm[q] = q;
+  const int step = incp > 0 ? 1 : -1;
+  const int ib = incp > 0 ? k1 : k2;
+  const int ie = incp > 0 ? k2 + 1 : k1 - 1;
+  for (int i = ib; i != ie; i += step) {
+    if (i < 0 || i >= rows) break;
+    const int t = i + hp[(size_t)(i - k1) * pstride];
+    if (t < 0 || t >= rows) break;  // pivot outside the matrix: leave the rest alone
+    const int a = perm[i];
+    perm[i] = perm[t];
+    perm[t] = a;
+  }
   void* w = nullptr;
   const size_t perm_bytes = ((size_t)rows * sizeof(int) + 255) & ~(size_t)255;
   FLA_TRY(fla_workspace(perm_bytes + (size_t)rows * ncols * sizeof(double), &w));
-  int* perm = static_cast<int*>(w);
+  int* dperm = static_cast<int*>(w);
   double* tmp = reinterpret_cast<double*>(static_cast<char*>(w) + perm_bytes);
-  hipLaunchKernelGGL(perm_init_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, rows, perm);
-  int* bad = fla_ctx().dscratch + 8;
-  FLA_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
-  hipLaunchKernelGGL(perm_walk_kernel, dim3(1), dim3(64), 0, s, rows, k1, k2, p, incp, perm, bad);
+  FLA_HIP_CHECK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * rows, hipMemcpyHostToDevice, s));
   // tmp := A (rows x ncols), then A(q, :) := tmp(perm[q], :)
   FLA_HIP_CHECK(hipMemcpy2DAsync(tmp, (size_t)rows * sizeof(double), A, (size_t)lda * sizeof(double),
                                  (size_t)rows * sizeof(double), (size_t)ncols,
                                  hipMemcpyDeviceToDevice, s));
   const long total = (long)rows * ncols;
   hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     rows, ncols, perm, tmp, (long)rows, A, (long)lda);
+                     rows, ncols, dperm, tmp, (long)rows, A, (long)lda);
   FLA_HIP_CHECK(hipGetLastError());
+  // perm (host) must outlive the upload
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
   return FLA_HIP_OK;
 }
 
