@@ -375,6 +375,43 @@ def bench_adjoint(fla, np, n):
     out = {"n": n, "ms": round(t * 1e3, 2), "tflops_declared": round(flops / t / 1e12, 3),
            "lml": lml.value, "grad_l": g[0], "grad_s": g[1], "info": info.value}
     log(f"adjoint: {out}")
+    out["batched_fp32"] = bench_adjoint_batched(fla, np, 1024, 256)
+    return out
+
+
+def bench_adjoint_batched(fla, np, count, n):
+    """Batched FP32 GP gradient: count problems of n points in [0,1]^2, l_i
+    uniform on [0.05, 0.5], s = 1, noise 1e-3, y ~ N(0,1) (one launch each for
+    the kernel matrices, the batched Cholesky + solve and the batched
+    adjoint + contraction).  Declared flops per problem as for the single
+    problem: n^3/3 + n^3/3 + 2 n^3."""
+    lib = fla.lib()
+    rng = np.random.default_rng(1024)
+    x = rng.uniform(0.0, 1.0, size=count * n * 2).astype(np.float32)
+    y = rng.standard_normal(count * n).astype(np.float32)
+    ls = rng.uniform(0.05, 0.5, size=count).astype(np.float32)
+    dx = fla.DeviceMatrix.from_host(x.reshape(-1, 1), np.float32)
+    dy = fla.DeviceMatrix.from_host(y.reshape(-1, 1), np.float32)
+    dl = fla.DeviceMatrix.from_host(ls.reshape(-1, 1), np.float32)
+    dlml = fla.DeviceMatrix(count, 1, np.float32)
+    dg = fla.DeviceMatrix(2 * count, 1, np.float32)
+    dinfo = fla.DeviceMatrix(count, 1, np.int32)
+    ts = []
+    for rep in range(4):
+        fla.synchronize()
+        t0 = time.perf_counter()
+        fla.check(lib.fla_hip_sgp_rbf_lml_grad_batched(count, n, 2, dx.ptr, dy.ptr, dl.ptr, 1.0,
+                                                       1e-3, dlml.ptr, dg.ptr, dinfo.ptr))
+        fla.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t = min(ts[1:])
+    info = dinfo.download().ravel()
+    flops = count * (n ** 3 / 3 + n ** 3 / 3 + 2.0 * n ** 3)
+    out = {"count": count, "n": n, "ms": round(t * 1e3, 3),
+           "tflops_declared": round(flops / t / 1e12, 3),
+           "failed": int(np.sum(info != fla.FLA_SUCCESS)),
+           "lml_mean": float(np.nanmean(dlml.download()))}
+    log(f"adjoint batched fp32: {out}")
     return out
 
 

@@ -229,6 +229,23 @@ int fla_hip_gp_rbf_grad(int n, int d, const double* x, int ldx, double l, double
                         const double* Kbar, int ldkb, double* g);
 int fla_hip_gp_rbf_lml_grad(int n, int d, const double* x, int ldx, const double* y, double l,
                             double s, double noise, double* lml, double* grad, int* info);
+/* Batched FP32 Cholesky adjoint (no counterpart in the reference): for
+   count problems of order n <= 256 stored as consecutive column-major n x n
+   blocks (ld n), Abar_i = Phi-sym(L_i^-T Phi(L_i^T Lbar_i) L_i^-1); only the
+   lower triangles of L_i and Lbar_i are read.  One workgroup per problem. */
+int fla_hip_schol_adjoint_batched(int count, int n, const float* L, const float* Lbar,
+                                  float* Abar);
+/* Batched FP32 GP log marginal likelihood and its gradient in (l_i, s) for
+   count problems with n <= 256 points in d <= 8 dimensions: x = count
+   consecutive n x d column-major blocks (ld n), y = count vectors of n,
+   l = count length scales; K_i = s^2 exp(-|x-x'|^2/(2 l_i^2)) + noise I is
+   factored by fla_hip_sposv_batched, Kbar_i assembled by the batched
+   Cholesky adjoint (Lbar = -diag(1/L_jj)) plus the Cholesky-solve adjoint.
+   Device outputs: lml[count], grad[2*count] = (d/dl_i, d/ds) pairs, and
+   info[count] (FLA_SUCCESS, or the failing column with NaN outputs). */
+int fla_hip_sgp_rbf_lml_grad_batched(int count, int n, int d, const float* x, const float* y,
+                                     const float* l, float s, float noise, float* lml,
+                                     float* grad, int* info);
 
 /* ---- synthetic inputs and checks (device side) ---- */
 /* Counter-based generators: identical values to the CPU generators. */

@@ -140,6 +140,9 @@ def lib():
             "fla_hip_dsgesv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int,
                                          vp, C.c_int, C.POINTER(MpOpts), ip, ip]),
             "fla_hip_mp_lu": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp, ip]),
+            "fla_hip_schol_adjoint_batched": (C.c_int, [C.c_int, C.c_int, vp, vp, vp]),
+            "fla_hip_sgp_rbf_lml_grad_batched": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                                           C.c_float, C.c_float, vp, vp, vp]),
             "fla_hip_gen_ge_reflectors": (C.c_int, [C.c_ulonglong, C.c_ulonglong, C.c_int,
                                                     C.c_int, C.c_double, vp, C.c_int]),
             "fla_hip_mp_last_stats": (C.c_int, [dp, C.c_int]),
