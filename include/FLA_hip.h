@@ -124,6 +124,8 @@ typedef struct {
   int max_iter;      /* refinement steps before the FP64 fall-back (default 30) */
   int gmres_restart; /* Krylov dimension per GMRES-IR step (default 20) */
   double gmres_tol;  /* relative preconditioned residual per GMRES-IR step (default 1e-4) */
+  double stop_factor; /* stop when max|r_j| <= max|x_j| * ||A||_inf * eps * stop_factor for
+                         every column; 0 (default) = sqrt(n), dsposv_'s rule */
 } fla_hip_mp_opts;
 /* Reduced-precision Cholesky of the symmetric A (uplo triangle): trailing
    updates in FP32 or BF16 on MFMA, each panel factored in FP64 from the

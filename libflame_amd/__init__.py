@@ -169,7 +169,8 @@ def lib():
 class MpOpts(C.Structure):
     """fla_hip_mp_opts (FLA_hip.h)."""
     _fields_ = [("precision", C.c_int), ("refinement", C.c_int), ("max_iter", C.c_int),
-                ("gmres_restart", C.c_int), ("gmres_tol", C.c_double)]
+                ("gmres_restart", C.c_int), ("gmres_tol", C.c_double),
+                ("stop_factor", C.c_double)]
 
 
 FLA_MP_FP32, FLA_MP_BF16 = 0, 1
@@ -178,9 +179,9 @@ FLA_HIP_ERANGE = 9004
 
 
 def dsposv(uplo, A, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_iter=30,
-           gmres_restart=20, gmres_tol=1e-4):
+           gmres_restart=20, gmres_tol=1e-4, stop_factor=0.0):
     """fla_hip_dsposv on device matrices; returns (iter, info, stats dict)."""
-    o = MpOpts(precision, refinement, max_iter, gmres_restart, gmres_tol)
+    o = MpOpts(precision, refinement, max_iter, gmres_restart, gmres_tol, stop_factor)
     it, info = C.c_int(), C.c_int()
     check(lib().fla_hip_dsposv(uplo, A.m, B.n, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld,
                                C.byref(o), C.byref(it), C.byref(info)), "fla_hip_dsposv")
@@ -192,10 +193,10 @@ def dsposv(uplo, A, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_
 
 
 def dsgesv(A, p, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_iter=30,
-           gmres_restart=20, gmres_tol=1e-4):
+           gmres_restart=20, gmres_tol=1e-4, stop_factor=0.0):
     """fla_hip_dsgesv on device matrices (p: device int vector of n); returns
     (iter, info, stats dict)."""
-    o = MpOpts(precision, refinement, max_iter, gmres_restart, gmres_tol)
+    o = MpOpts(precision, refinement, max_iter, gmres_restart, gmres_tol, stop_factor)
     it, info = C.c_int(), C.c_int()
     check(lib().fla_hip_dsgesv(A.m, B.n, A.ptr, A.ld, p.ptr, B.ptr, B.ld, X.ptr, X.ld,
                                C.byref(o), C.byref(it), C.byref(info)), "fla_hip_dsgesv")

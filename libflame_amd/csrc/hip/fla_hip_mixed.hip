@@ -940,12 +940,13 @@ static int mp_solve(bool general, int uplo, int n, int nrhs, double* A, int lda,
   o.max_iter = 30;
   o.gmres_restart = 20;
   o.gmres_tol = 1e-4;
+  o.stop_factor = 0.0;
   if (opts) o = *opts;
   if (!general && uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
   if (o.precision != FLA_MP_FP32 && o.precision != FLA_MP_BF16) return FLA_HIP_EARG;
   if (o.refinement != FLA_IR_CLASSIC && o.refinement != FLA_IR_GMRES) return FLA_HIP_EARG;
   if (n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1) ||
-      ldx < (n > 1 ? n : 1) || o.max_iter < 0 || o.gmres_restart < 1)
+      ldx < (n > 1 ? n : 1) || o.max_iter < 0 || o.gmres_restart < 1 || !(o.stop_factor >= 0.0))
     return FLA_HIP_EARG;
   if (general && n > 0 && !piv) return FLA_HIP_EARG;
   *iter = 0;
@@ -1035,7 +1036,7 @@ static int mp_solve(bool general, int uplo, int n, int nrhs, double* A, int lda,
   FLA_HIP_CHECK(hipMemcpyAsync(&anrm, st.hbuf, sizeof(double), hipMemcpyDeviceToHost, s));
   FLA_HIP_CHECK(hipStreamSynchronize(s));
   const double eps = DBL_EPSILON * 0.5;
-  const double cte = anrm * eps * std::sqrt((double)n);
+  const double cte = anrm * eps * (o.stop_factor > 0.0 ? o.stop_factor : std::sqrt((double)n));
 
   int code = 0;  // 0 ok, -2 overflow, -3 factor failed, -31 no convergence
   {

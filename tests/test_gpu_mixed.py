@@ -166,6 +166,17 @@ def test_dsposv_fp32_classic(orc, kappa, uplo):
     assert fwd <= 50 * kappa * EPS
 
 
+@pytest.mark.parametrize("refinement", [fla.FLA_IR_CLASSIC, fla.FLA_IR_GMRES])
+def test_dsposv_stop_factor(orc, refinement):
+    """A tighter stopping factor is honoured: backward error <= 4 eps."""
+    n = 1024
+    A = dev_reflector_spd(7, n, 64, 1e6).download()
+    B = orc.gen_normal(98, n, 8)
+    it, info, st, X, _ = run_dsposv(A, B, refinement=refinement, stop_factor=4.0)
+    assert info == 0 and 0 <= it <= 30, it
+    assert backward_error(A, X, B) <= 4 * EPS
+
+
 @pytest.mark.parametrize("kappa", [1e2, 1e4])
 def test_dsposv_bf16_gmres(orc, kappa):
     n = 1024
