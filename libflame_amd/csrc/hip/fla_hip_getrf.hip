@@ -34,6 +34,7 @@ constexpr unsigned long long kSpinTicks = 200000000ULL;  // 2 s of the 100 MHz w
 //   [3] last NaN row (or -1)                  [4] its orig
 //   [5] best |v| over non-NaN rows > last NaN [6] its row (or -1)  [7] its orig
 constexpr int HDR = 8;
+constexpr int MAX_P = 256;  // panel workgroups (one per CU at most)
 
 struct PanelComm {
   double* hdr;      // [2][P][HDR]
@@ -85,12 +86,12 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
 
   __shared__ double prow[PW];   // pivot row values
   __shared__ double jrow[PW];   // old diagonal row values
-  __shared__ double lv[ROWS_PER_WG];
   __shared__ double cand[3][PW];
+  __shared__ int jorig;
+  __shared__ double lv[ROWS_PER_WG];
   __shared__ double wred[2][8];
   __shared__ double hsum[8];
   __shared__ int sel[4];        // winner row, winner orig, orig of the old diagonal row
-  __shared__ int jorig;
 
   // Thread owns rows base + rh*128 + 64*q + lane (q = 0..1), columns 32*cg + c.
   double v[RQ][32];
@@ -206,19 +207,29 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     }
     __syncthreads();
     const int br = (int)hsum[1], nr = (int)hsum[3], ar = (int)hsum[6];
-    // Stage candidate rows (and the diagonal row j) in LDS.
+    // Stage candidate rows (and the diagonal row j) in LDS: one branch per
+    // row around its 32 stores; waves holding none skip (a uniform test).
 #pragma unroll
     for (int q = 0; q < RQ; ++q) {
       const int r = base + rh * 128 + 64 * q + lane;
+      if (!__any(r == br || r == nr || r == ar || r == j)) continue;
+      if (r == br) {
 #pragma unroll
-      for (int c = 0; c < 32; ++c) {
-        const int col = 32 * cg + c;
-        if (r == br) cand[0][col] = v[q][c];
-        if (r == nr) cand[1][col] = v[q][c];
-        if (r == ar) cand[2][col] = v[q][c];
-        if (r == j) jrow[col] = v[q][c];
+        for (int c = 0; c < 32; ++c) cand[0][32 * cg + c] = v[q][c];
       }
-      if (r == j && cg == 0) jorig = orig[q];
+      if (r == nr) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) cand[1][32 * cg + c] = v[q][c];
+      }
+      if (r == ar) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) cand[2][32 * cg + c] = v[q][c];
+      }
+      if (r == j) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) jrow[32 * cg + c] = v[q][c];
+        if (cg == 0) jorig = orig[q];
+      }
     }
     __syncthreads();
     // ---- wave 0: publish this workgroup's slot for column j, wait for every
@@ -261,10 +272,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         double gnr = -1.0, gno = -1.0;
         for (unsigned w = lane; w < P; w += 64) {
           const double* h = cm.hdr + ((long)par * P + w) * HDR;
-          const double hn = ld_d(&h[3]);
+          const double hn = ld_d(&h[3]), ho = ld_d(&h[4]);  // one round trip
           if (hn > gnr) {
             gnr = hn;
-            gno = ld_d(&h[4]);
+            gno = ho;
           }
         }
         for (int off = 32; off > 0; off >>= 1) {
@@ -355,6 +366,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
         const int rr = base + rh * 128 + 64 * q + lane;
+        if (!__any(rr == j || rr == r)) continue;  // this wave holds neither row
         if (rr == j) {
 #pragma unroll
           for (int c = 0; c < 32; ++c) v[q][c] = prow[32 * cg + c];
@@ -389,17 +401,18 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     }
     __syncthreads();
     if (32 * cg + 31 > j) {
+      // Branch-free (results selected), so the prow loads are batched.
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
         const int lr = rh * 128 + 64 * q + lane;
         const int rr = base + lr;
-        if (rr > j && rr < M) {
-          const double l = lv[lr];
+        const bool act = rr > j && rr < M;
+        const double l = lv[lr];
 #pragma unroll
-          for (int c = 0; c < 32; ++c) {
-            const int col = 32 * cg + c;
-            if (col > j && col < W) v[q][c] -= l * prow[col];
-          }
+        for (int c = 0; c < 32; ++c) {
+          const int col = 32 * cg + c;
+          const double u = v[q][c] - l * prow[col];
+          v[q][c] = (act && col > j && col < W) ? u : v[q][c];
         }
       }
     }
@@ -564,7 +577,7 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
   const int nb = c.lu_nb <= PW ? c.lu_nb : PW;
   const int Pmax = (m + ROWS_PER_WG - 1) / ROWS_PER_WG;
   // Every panel workgroup must be resident at once (one per CU at most).
-  if (Pmax > c.num_cus) {
+  if (Pmax > c.num_cus || Pmax > MAX_P) {
     std::fprintf(stderr, "libflame_amd: dgetrf supports m <= %d on this device\n",
                  c.num_cus * ROWS_PER_WG);
     return FLA_HIP_EARG;
@@ -636,7 +649,7 @@ static LuBufs g_mp_bufs[2];
 int fla_lu_panel_setup(int m, int* derr) {
   FlaHipCtx& c = fla_ctx();
   const int Pmax = (m + ROWS_PER_WG - 1) / ROWS_PER_WG;
-  if (Pmax > c.num_cus) {
+  if (Pmax > c.num_cus || Pmax > MAX_P) {
     std::fprintf(stderr, "libflame_amd: the LU panel supports m <= %d on this device\n",
                  c.num_cus * ROWS_PER_WG);
     return FLA_HIP_EARG;
