@@ -35,7 +35,7 @@ constexpr int DLD = PB + 1;  // LDS leading dimension of the diagonal block
 //      rule, l_ij = a_ij * (1/d), a_ic -= l_ij l_cj;
 //   2. the rows below are solved against it (lane = row, row in VGPRs, the
 //      32x32 factor read from LDS by broadcast), dtrsm's update order;
-//   3. the trailing part is updated with 4x4 register tiles (rank 32).
+//   3. the trailing part is updated on MFMA (16x16 lower tiles, rank 32).
 // The block lives in LDS (129 KB); padding beyond nb is the identity.
 // INV: after a successful factor, also writes inv(L) (lower, ld 128, zero
 // upper part) to W, inverted in LDS (fla_hip_lds_tri.h).
@@ -86,11 +86,10 @@ __global__ __launch_bounds__(256) void potrf_blk_kernel(int nb, double* __restri
         const double inv = 1.0 / d;
         if ((lane & 31) > j) x[j] *= inv;
         if ((lane & 31) == j) x[j] = d;
+        // entries above the diagonal (lane < c) take garbage here and are
+        // never stored or read, so the update needs no per-lane predicate
 #pragma unroll
-        for (int c = j + 1; c < LB; ++c) {
-          const double lcj = readlane_d(x[j], c);
-          if ((lane & 31) >= c) x[c] -= x[j] * lcj;
-        }
+        for (int c = j + 1; c < LB; ++c) x[c] -= x[j] * readlane_d(x[j], c);
       }
       if (lane < 32) {
 #pragma unroll
@@ -118,35 +117,32 @@ __global__ __launch_bounds__(256) void potrf_blk_kernel(int nb, double* __restri
       for (int c = 0; c < LB; ++c) S[(b + c) * DLD + r] = x[c];
     }
     __syncthreads();
-    // 3. trailing update: A(i, c) -= sum_k L(i, k) L(c, k), i >= c >= b + 32
-    const int T4 = nrow / 4;
-    const int ntile = T4 * (T4 + 1) / 2;
-    for (int tt = t; tt < ntile; tt += 256) {
-      int I = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-      while (I * (I + 1) / 2 > tt) --I;
-      while ((I + 1) * (I + 2) / 2 <= tt) ++I;
-      const int J = tt - I * (I + 1) / 2;
-      const int i0 = b + LB + 4 * I, c0 = b + LB + 4 * J;
-      double acc[4][4] = {};
-#pragma unroll 8
-      for (int k = 0; k < LB; ++k) {
-        const double* col = S + (b + k) * DLD;
-        double li[4], lc[4];
+    // 3. trailing update A(i, c) -= sum_k L(i, k) L(c, k), i >= c >= b + 32,
+    //    on v_mfma_f64_16x16x4f64: 16 x 16 lower tiles spread over the waves;
+    //    lane (li, lk), register q holds D(16 ti + li, 16 tj + lk + 4 q).
+    {
+      const int T16 = nrow / 16;
+      const int ntile = T16 * (T16 + 1) / 2;
+      const int i0 = b + LB;
+      const int li = lane & 15, lk = lane >> 4;
+      for (int tt = wave; tt < ntile; tt += 4) {
+        int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
+        while (ti * (ti + 1) / 2 > tt) --ti;
+        while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
+        const int tj = tt - ti * (ti + 1) / 2;
+        fla_lds::d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          li[q] = col[i0 + q];
-          lc[q] = col[c0 + q];
+        for (int ks = 0; ks < LB / 4; ++ks) {
+          const double* col = S + (b + 4 * ks + lk) * DLD;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(col[i0 + 16 * tj + li], col[i0 + 16 * ti + li],
+                                                     acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
-#pragma unroll
-          for (int y = 0; y < 4; ++y) acc[x][y] += li[x] * lc[y];
+        for (int q = 0; q < 4; ++q) {
+          const int row = i0 + 16 * ti + li, cc = i0 + 16 * tj + lk + 4 * q;
+          if (row >= cc) S[cc * DLD + row] -= acc[q];
+        }
       }
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-          if (i0 + x >= c0 + y) S[(c0 + y) * DLD + i0 + x] -= acc[x][y];
     }
     __syncthreads();
   }
