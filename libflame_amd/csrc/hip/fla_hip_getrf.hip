@@ -15,8 +15,10 @@
 // workgroups (256 rows each) and held in VGPRs for the whole panel; every
 // column costs one grid-wide barrier, across which each workgroup publishes
 // its local pivot candidate (and that row's values) and the owner of the
-// diagonal row publishes that row.  All inter-workgroup traffic uses
-// agent-scope atomics, and every wait is bounded.
+// diagonal row publishes that row.  All inter-workgroup traffic goes through
+// uncached device memory with agent-scope atomic loads and stores (so no L2
+// write-back or invalidate is needed across the XCDs), and every wait is
+// bounded.
 #include <hip/hip_runtime.h>
 
 #include "fla_hip_internal.h"
@@ -244,7 +246,11 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         if (owns_j) st_d(&jslot[e], jrow[e]);
       }
       if (owns_j && lane == 0) st_d(&jslot[PW], (double)jorig);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      // The exchange buffers are uncached: waiting for the stores to be
+      // acknowledged orders them before the sequence number, with no L2
+      // write-back.
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane == 0)
         __hip_atomic_store(&cm.seq[par * P + wg], j + 1, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // uncached: no invalidate
       if (!ok) {
         if (lane == 0) sel[0] = -2;
       } else {
@@ -477,24 +483,6 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 }  // namespace
 
-struct LuScratch {
-  void* buf = nullptr;
-  size_t bytes = 0;
-};
-static LuScratch g_lu;
-
-static int lu_scratch(size_t bytes, void** p) {
-  if (bytes > g_lu.bytes) {
-    FLA_HIP_CHECK(hipDeviceSynchronize());
-    if (g_lu.buf) FLA_HIP_CHECK(hipFree(g_lu.buf));
-    g_lu.buf = nullptr;
-    FLA_HIP_CHECK(hipMalloc(&g_lu.buf, bytes));
-    g_lu.bytes = bytes;
-  }
-  *p = g_lu.buf;
-  return FLA_HIP_OK;
-}
-
 // Buffers for one panel launch (two sets, alternated between steps).
 struct LuBufs {
   PanelComm cm;
@@ -582,11 +570,11 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
                  c.num_cus * ROWS_PER_WG);
     return FLA_HIP_EARG;
   }
-  const size_t set = align256(lu_set_bytes(Pmax));
-  void* sp = nullptr;
-  FLA_TRY(lu_scratch(2 * set, &sp));
-  LuBufs bufs[2] = {lu_carve(static_cast<char*>(sp), Pmax),
-                    lu_carve(static_cast<char*>(sp) + set, Pmax)};
+  // exchange buffers in the uncached region, laid out for MAX_P workgroups
+  const size_t set = align256(lu_set_bytes(MAX_P));
+  static_assert(2 * (8 * 2 * MAX_P * HDR + 8 * 2 * MAX_P * 3 * PW) < kUcLuBytes, "LU exchange");
+  char* sp = c.uc;
+  LuBufs bufs[2] = {lu_carve(sp, MAX_P), lu_carve(sp + set, MAX_P)};
   int* dinfo = c.dscratch;
   int* derr = c.dscratch + 4;  // one wait-failure flag for the whole factorization
   bufs[0].cm.err = derr;
@@ -654,11 +642,9 @@ int fla_lu_panel_setup(int m, int* derr) {
                  c.num_cus * ROWS_PER_WG);
     return FLA_HIP_EARG;
   }
-  const size_t set = align256(lu_set_bytes(Pmax > 0 ? Pmax : 1));
-  void* sp = nullptr;
-  FLA_TRY(lu_scratch(2 * set, &sp));
-  g_mp_bufs[0] = lu_carve(static_cast<char*>(sp), Pmax);
-  g_mp_bufs[1] = lu_carve(static_cast<char*>(sp) + set, Pmax);
+  const size_t set = align256(lu_set_bytes(MAX_P));
+  g_mp_bufs[0] = lu_carve(c.uc, MAX_P);
+  g_mp_bufs[1] = lu_carve(c.uc + set, MAX_P);
   g_mp_bufs[0].cm.err = derr;
   g_mp_bufs[1].cm.err = derr;
   return FLA_HIP_OK;

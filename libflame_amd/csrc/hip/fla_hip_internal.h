@@ -44,12 +44,24 @@ struct FlaHipCtx {
   size_t work_bytes[kSlots] = {};
   // Small persistent device scratch: [0] info word, [1..] counters.
   int* dscratch = nullptr;
+  // Device memory the GPU does not cache (hipDeviceMallocUncached) for data
+  // that workgroups exchange inside one kernel: stores and loads of it need
+  // no L2 write-back or invalidate across the XCDs.  Allocated once and kept
+  // until the process ends (this kind of memory must not be freed after a
+  // kernel read it).  [0, kUcLuBytes): the LU panel's two exchange buffer
+  // sets; then the triangular solve's copy of the finished right-hand-side
+  // blocks (16 columns of n <= kUcSolveRows rows).
+  char* uc = nullptr;
   // 128 x 128 FP64 block: inverse of the current Cholesky diagonal block.
   double* dlinv = nullptr;
   int chol_nb = 0;  // 0: chosen from n (fla_chol_nb)
   int mp_nb = 0;    // reduced-precision factor block; 0: fla_chol_nb
   int lu_nb = 128;
 };
+
+constexpr size_t kUcLuBytes = 4u << 20;
+constexpr long kUcSolveRows = 131072;
+constexpr size_t kUcBytes = kUcLuBytes + (size_t)kUcSolveRows * 16 * sizeof(double);
 
 FlaHipCtx& fla_ctx();
 // Outer Cholesky block for order n: the configured one, else n/32 rounded

@@ -77,6 +77,12 @@ extern "C" int fla_hip_init(void) {
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
   FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dscratch), 4096));
   FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dlinv), 128 * 128 * sizeof(double)));
+  // uncached exchange memory: allocated once per process, never freed
+  static char* uc = nullptr;
+  if (!uc)
+    FLA_HIP_CHECK(hipExtMallocWithFlags(reinterpret_cast<void**>(&uc), kUcBytes,
+                                        hipDeviceMallocUncached));
+  g_ctx.uc = uc;
   g_ctx.ready = true;
   return FLA_HIP_OK;
 }

@@ -7,11 +7,20 @@
 // J it depends on as soon as block J is final, then multiplies by the inverted
 // diagonal block (staged in LDS at launch), writes X_I and publishes it.
 // Blocks finish in dependency order, so one monotonic counter says which are
-// final (block s of the processing order is final when done > s).  The T tile
-// of the next dependency is loaded into registers before waiting, so the wait
-// hides its HBM latency; the triangle is read from HBM once per solve.
-// Every wait is bounded by the 100 MHz wall clock; an expired wait sets *err
-// and the kernel drains.
+// final (block s of the processing order is final when done > s).  Every
+// wait is bounded by the 100 MHz wall clock; an expired wait sets *err and
+// the kernel drains.
+//
+// Twelve waves per workgroup: eight compute waves own 16 rows each and hold
+// the op(T)_IJ fragments of the current dependency and of the next one
+// (loaded one step ahead, so the triangle streams from HBM while the chain
+// waits); four loader waves poll the counter and stage X_J in LDS.  The
+// roles matter because a wave's vector-memory counter completes in order:
+// X_J loads issued behind a wave's in-flight triangle loads would wait for
+// all of them.  The triangle is read from HBM once per solve.  Finished
+// blocks are exchanged through an uncached copy Xm (rows x 16), so neither
+// side needs an L2 write-back or invalidate across the XCDs: the writer waits
+// for its stores to be acknowledged before bumping the counter.
 //
 // Products use v_mfma_f64_16x16x4f64 with the right-hand side as the MFMA A
 // operand and the triangle as B, so D(col, row) lands in lane
@@ -25,17 +34,47 @@ namespace {
 
 constexpr int NB = 128;
 constexpr int RHS = 16;
+constexpr int NCOMP = 8;                 // compute waves (16 rows each)
+constexpr int NTHREADS = (NCOMP + 4) * 64;  // + four loader waves
 constexpr unsigned long long kWaitTicks = 200000000ULL;  // 2 s
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// op(T)(row, c0 + 32*lk + ks) for ks = 0..31 of this lane's row.
+template <bool TRANS>
+__device__ __forceinline__ void load_frag(double (&f)[32], const double* __restrict__ T, long ldt,
+                                          int n, int row, int c0, int lk, bool full) {
+  const int kb = c0 + 32 * lk;
+  if (TRANS) {
+    // the lane's 32 elements are contiguous: 16-byte loads when aligned
+    const double* p = T + (long)kb + (long)row * ldt;
+    if (full && ((reinterpret_cast<unsigned long>(p) & 15) == 0)) {
+#pragma unroll
+      for (int ks = 0; ks < 32; ks += 2) {
+        const d2 v = *reinterpret_cast<const d2*>(p + ks);
+        f[ks] = v[0];
+        f[ks + 1] = v[1];
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks) f[ks] = (row < n && kb + ks < n) ? p[ks] : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int ks = 0; ks < 32; ++ks)
+      f[ks] = (full || (row < n && kb + ks < n)) ? T[(long)row + (long)(kb + ks) * ldt] : 0.0;
+  }
+}
 
 template <bool FWD, bool TRANS>
-__global__ __launch_bounds__(256) void trsm_left_narrow_kernel(int n, int nrhs,
-                                                               const double* __restrict__ T,
-                                                               long ldt,
-                                                               const double* __restrict__ W,
-                                                               double* X, long ldx, int* done,
-                                                               int* err) {
+__global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int nrhs,
+                                                                    const double* __restrict__ T,
+                                                                    long ldt,
+                                                                    const double* __restrict__ W,
+                                                                    double* X, long ldx,
+                                                                    double* __restrict__ Xm,
+                                                                    int* done, int* err) {
   __shared__ double Wf[NB * NB];   // op(inv(T_II)), fragment order
   __shared__ double Xf[NB * RHS];  // X_J (negated) or Y_I, fragment order
   __shared__ int bail;
@@ -43,47 +82,43 @@ __global__ __launch_bounds__(256) void trsm_left_narrow_kernel(int n, int nrhs,
   const int I = FWD ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
   const int r0 = I * NB;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+  const bool comp = w < NCOMP;
+  const int lt = t - NCOMP * 64;  // loader thread index (0..255) when !comp
 
   // op(W_I)(row, k) -> Wf[((row/16)*32 + k%32)*64 + (k/32)*16 + row%16]
   const double* Wi = W + (long)I * NB * NB;
-  for (int e = t; e < NB * NB; e += 256) {
+  for (int e = t; e < NB * NB; e += NTHREADS) {
     const int p = e & (NB - 1), q = e >> 7;  // stored W(p, q)
     const int row = TRANS ? q : p, k = TRANS ? p : q;
     Wf[((row >> 4) * 32 + (k & 31)) * 64 + (k >> 5) * 16 + (row & 15)] = Wi[e];
   }
   if (t == 0) bail = 0;
 
-  // acc[a] holds X(r0 + 32w + 16a + li, lk + 4r) in component r.
-  d4 acc[2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int row = r0 + 32 * w + 16 * a + li;
+  // compute wave w: acc holds X(r0 + 16w + li, lk + 4r) in component r.
+  const int row = r0 + 16 * w + li;
+  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  if (comp) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int col = lk + 4 * r;
-      acc[a][r] = (row < n && col < nrhs) ? X[(long)row + (long)col * ldx] : 0.0;
+      acc[r] = (row < n && col < nrhs) ? X[(long)row + (long)col * ldx] : 0.0;
     }
   }
 
   const int ndep = FWD ? I : nb - 1 - I;
-  for (int s = 0; s < ndep; ++s) {
-    const int J = FWD ? s : nb - 1 - s;
-    const int c0 = J * NB;
-    // op(T)_IJ fragments: tf[a][ks] = op(T)(r0 + 32w + 16a + li, c0 + 32lk + ks).
-    // They do not depend on other workgroups: load them before waiting.
-    double tf[2][32];
-    const bool full = (r0 + NB <= n) && (c0 + NB <= n);
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int row = r0 + 32 * w + 16 * a + li;
-#pragma unroll
-      for (int ks = 0; ks < 32; ++ks) {
-        const int k = c0 + 32 * lk + ks;
-        const long off = TRANS ? (long)k + (long)row * ldt : (long)row + (long)k * ldt;
-        tf[a][ks] = (full || (row < n && k < n)) ? T[off] : 0.0;
-      }
+  auto dep_col = [&](int s) { return (FWD ? s : nb - 1 - s) * NB; };
+  auto full_tile = [&](int c0) { return (r0 + NB <= n) && (c0 + NB <= n); };
+  double fa[32], fb[32];
+  if (comp && ndep > 0) load_frag<TRANS>(fa, T, ldt, n, row, dep_col(0), lk, full_tile(dep_col(0)));
+
+  // One dependency: prefetch the next fragments, wait for X_J, stage it, fold it in.
+  auto step = [&](int s, double (&cur)[32], double (&nxt)[32]) -> bool {
+    const int c0 = dep_col(s);
+    if (comp && s + 1 < ndep) {
+      const int c1 = dep_col(s + 1);
+      load_frag<TRANS>(nxt, T, ldt, n, row, c1, lk, full_tile(c1));
     }
-    if (t == 0) {
+    if (lt == 0) {
       const unsigned long long t0 = wall_clock64();
       while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= s) {
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
@@ -96,55 +131,59 @@ __global__ __launch_bounds__(256) void trsm_left_narrow_kernel(int n, int nrhs,
       }
     }
     __syncthreads();
-    if (bail) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int e = t; e < NB * RHS; e += 256) {
-      const int k = e & (NB - 1), col = e >> 7;
-      const double v = (c0 + k < n && col < nrhs) ? X[(long)(c0 + k) + (long)col * ldx] : 0.0;
-      Xf[(k & 31) * 64 + (k >> 5) * 16 + col] = -v;
+    if (bail) return false;
+    if (!comp) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // Xm is uncached
+      for (int e = lt; e < NB * RHS; e += 256) {
+        const int k = e & (NB - 1), col = e >> 7;
+        const double v = (c0 + k < n && col < nrhs)
+                             ? __hip_atomic_load(&Xm[(long)(c0 + k) + (long)col * n],
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : 0.0;
+        Xf[(k & 31) * 64 + (k >> 5) * 16 + col] = -v;
+      }
     }
     __syncthreads();
+    if (comp) {
 #pragma unroll
-    for (int ks = 0; ks < 32; ++ks) {
-      const double xa = Xf[ks * 64 + lane];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-        acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, tf[a][ks], acc[a], 0, 0, 0);
+      for (int ks = 0; ks < 32; ++ks)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], cur[ks], acc, 0, 0, 0);
     }
     __syncthreads();
+    return true;
+  };
+  for (int s = 0; s < ndep; s += 2) {
+    if (!step(s, fa, fb)) return;
+    if (s + 1 < ndep && !step(s + 1, fb, fa)) return;
   }
 
-  // Y_I -> Xf as the contraction operand (its row is the contraction index).
+  // Y_I -> Xf as the contraction operand (its row k = 16w + li is the
+  // contraction index), then X_I = op(W_I) Y_I.
+  if (comp) {
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Xf[(16 * a + li) * 64 + w * 16 + lk + 4 * r] = acc[a][r];
-  __syncthreads();
-  d4 out[2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a) out[a] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int ks = 0; ks < 32; ++ks) {
-    const double ya = Xf[ks * 64 + lane];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-      out[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, Wf[((2 * w + a) * 32 + ks) * 64 + lane],
-                                                    out[a], 0, 0, 0);
+    for (int r = 0; r < 4; ++r) Xf[(16 * (w & 1) + li) * 64 + (w >> 1) * 16 + lk + 4 * r] = acc[r];
   }
+  __syncthreads();
+  if (comp) {
+    d4 out = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int row = r0 + 32 * w + 16 * a + li;
+    for (int ks = 0; ks < 32; ++ks)
+      out = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], Wf[(w * 32 + ks) * 64 + lane],
+                                                 out, 0, 0, 0);
     if (row < n) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int col = lk + 4 * r;
-        if (col < nrhs) X[(long)row + (long)col * ldx] = out[a][r];
+        if (col < nrhs) {
+          X[(long)row + (long)col * ldx] = out[r];
+          Xm[(long)row + (long)col * n] = out[r];
+        }
       }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // the Xm stores are acknowledged
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
-  if (t == 0) __hip_atomic_fetch_add(done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0) __hip_atomic_fetch_add(done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void reset2_kernel(int* a, int* b) {
@@ -154,9 +193,9 @@ __global__ void reset2_kernel(int* a, int* b) {
 
 template <bool FWD, bool TRANS>
 void launch_one(hipStream_t s, int nb, int na, int nr, const double* T, long ldt,
-                const double* W, double* X, long ldx, int* done, int* err) {
-  hipLaunchKernelGGL((trsm_left_narrow_kernel<FWD, TRANS>), dim3(nb), dim3(256), 0, s, na, nr, T,
-                     ldt, W, X, ldx, done, err);
+                const double* W, double* X, long ldx, double* Xm, int* done, int* err) {
+  hipLaunchKernelGGL((trsm_left_narrow_kernel<FWD, TRANS>), dim3(nb), dim3(NTHREADS), 0, s, na, nr,
+                     T, ldt, W, X, ldx, Xm, done, err);
 }
 
 }  // namespace
@@ -167,18 +206,25 @@ int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, in
                             const double* T, long ldt, const double* W, double* X, long ldx,
                             int* done, int* err) {
   const int nb = (na + NB - 1) / NB;
+  if (na > kUcSolveRows) {
+    std::fprintf(stderr, "libflame_amd: the narrow triangular solve supports n <= %ld\n",
+                 kUcSolveRows);
+    return FLA_HIP_EARG;
+  }
+  // the uncached block copy; launches on one stream run one after another
+  double* Xm = reinterpret_cast<double*>(fla_ctx().uc + kUcLuBytes);
   for (int c = 0; c < nrhs; c += RHS) {
     const int nr = (nrhs - c) < RHS ? (nrhs - c) : RHS;
     double* Xc = X + (long)c * ldx;
     hipLaunchKernelGGL(reset2_kernel, dim3(1), dim3(1), 0, s, done, err);
     if (op_lower && !trans)
-      launch_one<true, false>(s, nb, na, nr, T, ldt, W, Xc, ldx, done, err);
+      launch_one<true, false>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
     else if (op_lower)
-      launch_one<true, true>(s, nb, na, nr, T, ldt, W, Xc, ldx, done, err);
+      launch_one<true, true>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
     else if (!trans)
-      launch_one<false, false>(s, nb, na, nr, T, ldt, W, Xc, ldx, done, err);
+      launch_one<false, false>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
     else
-      launch_one<false, true>(s, nb, na, nr, T, ldt, W, Xc, ldx, done, err);
+      launch_one<false, true>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
     FLA_HIP_CHECK(hipGetLastError());
   }
   return FLA_HIP_OK;
