@@ -276,9 +276,17 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         if (lane == 0) sel[0] = -2;
       } else {
         double gnr = -1.0, gno = -1.0;
+        // the first slot's no-NaN candidate comes in the same round trip
+        double f0 = 0.0, f1 = -1.0, f2 = -1.0;
         for (unsigned w = lane; w < P; w += 64) {
           const double* h = cm.hdr + ((long)par * P + w) * HDR;
-          const double hn = ld_d(&h[3]), ho = ld_d(&h[4]);  // one round trip
+          const double hn = ld_d(&h[3]), ho = ld_d(&h[4]);
+          const double h0 = ld_d(&h[0]), h1 = ld_d(&h[1]), h2 = ld_d(&h[2]);
+          if (w == lane) {
+            f0 = h0;
+            f1 = h1;
+            f2 = h2;
+          }
           if (hn > gnr) {
             gnr = hn;
             gno = ho;
@@ -301,9 +309,9 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
           double cv = 0.0;
           int cr = -1, co = -1, cs = 0;
           if (gnr < 0.0) {
-            cv = ld_d(&h[0]);
-            cr = (int)ld_d(&h[1]);
-            co = (int)ld_d(&h[2]);
+            cv = (w == lane) ? f0 : ld_d(&h[0]);
+            cr = (int)((w == lane) ? f1 : ld_d(&h[1]));
+            co = (int)((w == lane) ? f2 : ld_d(&h[2]));
           } else {
             const int hn = (int)ld_d(&h[3]);
             if (hn == (int)gnr) {
