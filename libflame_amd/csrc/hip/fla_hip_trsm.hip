@@ -163,7 +163,7 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
   FLA_TRY(launch_trtri_blocks(s, na, A, lda, upper, unit, W));
   // op(A) is lower when (lower, N) or (upper, T): forward substitution.
   const bool op_lower = (!upper && !tr) || (upper && tr);
-  if (side == kLeft && n <= kNarrowRhs) {
+  if (side == kLeft && n <= kNarrowRhs && na <= kUcSolveRows) {
     int* flags = reinterpret_cast<int*>(W + (long)nblk * TB * TB);
     FLA_TRY(launch_trsm_left_narrow(s, op_lower, tr, na, n, A, lda, W, B, ldb, flags, flags + 1));
     int herr = 0;
