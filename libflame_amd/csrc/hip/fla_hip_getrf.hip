@@ -482,6 +482,24 @@ __global__ __launch_bounds__(256) void lu_row_gather_kernel(int ncols, T* __rest
   }
 }
 
+// dst(j, l) := src(l, j) for an r x c block (32 x 32 tiles through LDS).
+__global__ __launch_bounds__(256) void transpose_kernel(int r, int c, const double* __restrict__ src,
+                                                        long lds, double* __restrict__ dst,
+                                                        long ldd) {
+  __shared__ double tile[32][33];
+  const int l0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int q = ty; q < 32; q += 8) {
+    const int l = l0 + tx, j = j0 + q;
+    tile[q][tx] = (l < r && j < c) ? src[(long)l + (long)j * lds] : 0.0;
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int j = j0 + tx, l = l0 + q;
+    if (l < r && j < c) dst[(long)j + (long)l * ldd] = tile[tx][q];
+  }
+}
+
 __global__ void lu_reset_kernel(int* seq, int nseq, int* moved) {
   for (int i = threadIdx.x; i < nseq; i += blockDim.x) seq[i] = 0;
   if (threadIdx.x == 0) moved[0] = 0;
@@ -544,9 +562,11 @@ static int lu_gather(hipStream_t s, int ncols, double* A, long lda, const int* m
 }
 
 // Row interchanges, A12 solve and A22 update for columns [c0, c0+nc) after
-// the panel at (k, kb).
+// the panel at (k, kb).  U12 is copied transposed into Ut (ld ldu) so that
+// the update's right operand is read with j contiguous, as in the Cholesky
+// update (16-deep K chunks, coalesced loads).
 static int lu_update_cols(hipStream_t s, int m, double* A, long lda, int k, int kb, int c0,
-                          int nc, const int* moved) {
+                          int nc, const int* moved, double* Ut, long ldu) {
   if (nc <= 0) return FLA_HIP_OK;
   double* Ak = A + (long)k;  // row k
   FLA_TRY(lu_gather(s, nc, Ak + (long)c0 * lda, lda, moved));
@@ -555,8 +575,11 @@ static int lu_update_cols(hipStream_t s, int m, double* A, long lda, int k, int 
   FLA_TRY(launch_trsm_llnu(s, nc, kb, A11, lda, A12, lda));
   const int mr = m - k - kb;
   if (mr > 0) {
+    hipLaunchKernelGGL(transpose_kernel, dim3((kb + 31) / 32, (nc + 31) / 32), dim3(256), 0, s, kb,
+                       nc, A12, lda, Ut, ldu);
+    FLA_HIP_CHECK(hipGetLastError());
     void* tok = fla_stats_begin(s);
-    FLA_TRY(launch_dgemm_mfma(s, false, false, 0, mr, nc, kb, -1.0, A11 + kb, lda, A12, lda, 1.0,
+    FLA_TRY(launch_dgemm_mfma(s, false, true, 0, mr, nc, kb, -1.0, A11 + kb, lda, Ut, ldu, 1.0,
                               A + (long)(k + kb) + (long)c0 * lda, lda, nullptr));
     fla_stats_end(s, tok, 1, 2.0 * mr * (double)nc * kb);
   }
@@ -583,6 +606,11 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
   static_assert(2 * (8 * 2 * MAX_P * HDR + 8 * 2 * MAX_P * 3 * PW) < kUcLuBytes, "LU exchange");
   char* sp = c.uc;
   LuBufs bufs[2] = {lu_carve(sp, MAX_P), lu_carve(sp + set, MAX_P)};
+  // transposed U12 of one step (n x nb, ld n)
+  void* wut = nullptr;
+  FLA_TRY(fla_workspace((size_t)(n > 1 ? n : 1) * PW * sizeof(double), &wut));
+  double* Ut = static_cast<double*>(wut);
+  const long ldu = n > 1 ? n : 1;
   int* dinfo = c.dscratch;
   int* derr = c.dscratch + 4;  // one wait-failure flag for the whole factorization
   bufs[0].cm.err = derr;
@@ -613,11 +641,11 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
     const int k2 = k + kb;
     const int kb2 = (mn - k2) < nb ? (mn - k2) : nb;  // next panel width (0 at the end)
     // (a) the next panel's columns, so that panel can start
-    FLA_TRY(lu_update_cols(su, m, A, lda, k, kb, k2, kb2 > 0 ? kb2 : 0, b.moved));
+    FLA_TRY(lu_update_cols(su, m, A, lda, k, kb, k2, kb2 > 0 ? kb2 : 0, b.moved, Ut, ldu));
     hipEvent_t ready = next_event();
     FLA_HIP_CHECK(hipEventRecord(ready, su));
     // (b) the remaining columns
-    FLA_TRY(lu_update_cols(su, m, A, lda, k, kb, k2 + kb2, n - k2 - kb2, b.moved));
+    FLA_TRY(lu_update_cols(su, m, A, lda, k, kb, k2 + kb2, n - k2 - kb2, b.moved, Ut + kb2, ldu));
     if (kb2 <= 0) break;
     FLA_HIP_CHECK(hipStreamWaitEvent(sq, ready, 0));
     FLA_TRY(lu_panel(sq, m - k2, kb2, A + (long)k2 + (long)k2 * lda, lda, k2, p, dinfo,
