@@ -9,16 +9,21 @@
 // (src/lapack/util/app/piv/ln/flamec/FLA_Apply_pivots_ln_opt_var1.c:323), and
 // the null-pivot rule of FLA_LU_piv_opd_var4 (FLA_LU_piv_opt_var4.c:277-283,
 // 325-332): the first null pivot's index is returned, its interchange and
-// scaling are skipped, and the sweep continues.
+// scaling are skipped, the update uses the unchanged row j, and the sweep
+// continues.
 //
 // Panel: one cooperative kernel.  The (m-k) x nb panel is spread over P
 // workgroups (256 rows each) and held in VGPRs for the whole panel; every
-// column costs one grid-wide barrier, across which each workgroup publishes
-// its local pivot candidate (and that row's values) and the owner of the
-// diagonal row publishes that row.  All inter-workgroup traffic goes through
-// uncached device memory with agent-scope atomic loads and stores (so no L2
-// write-back or invalidate is needed across the XCDs), and every wait is
-// bounded.
+// column costs one all-to-all exchange: each workgroup publishes its local
+// pivot candidates (value, row, original row) and their rows, the owner of
+// the diagonal row publishes that row, and every workgroup reads all
+// candidates, picks the same pivot, and reads the two rows it needs.
+// The exchange words live in uncached device memory (no L2 write-back or
+// invalidate across the XCDs) and carry their own tags, so no store needs
+// to be acknowledged before another one and no separate flag is polled:
+// each 8-byte word holds a 32-bit payload and the 32-bit tag of the
+// (launch, column) it belongs to, and a reader takes a word only once its
+// tag is current.  Every wait is bounded.
 #include <hip/hip_runtime.h>
 
 #include "fla_hip_internal.h"
@@ -27,356 +32,416 @@ namespace {
 
 constexpr int PW = 128;           // maximum panel width
 constexpr int ROWS_PER_WG = 256;  // rows of the panel held by one workgroup
-constexpr int NT = 512;           // 8 waves: wave w owns columns [32(w&3), +32), row half w>>2
-constexpr int RQ = 2;             // rows per thread
+constexpr int NT = 512;  // 8 waves: wave w owns columns [32(w>>1), +32) of row half w&1
+constexpr int RQ = 2;    // rows per thread
 constexpr unsigned long long kSpinTicks = 200000000ULL;  // 2 s of the 100 MHz wall clock
-
-// Per-workgroup candidate header (all doubles so one slot is uniform).
-//   [0] best |v| over non-NaN rows >= j      [1] its row (or -1)  [2] its orig
-//   [3] last NaN row (or -1)                  [4] its orig
-//   [5] best |v| over non-NaN rows > last NaN [6] its row (or -1)  [7] its orig
-constexpr int HDR = 8;
 constexpr int MAX_P = 256;  // panel workgroups (one per CU at most)
+constexpr int HW = 8;       // header entries per workgroup slot (one 128-byte line)
 
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) volatile u64x2 gv_u64x2;  // global, volatile
+
+// Exchange area (tagged 16-byte entries, two parities: column j uses j & 1).
+//   hdr [2][P][HW]: {best |v| over non-NaN rows >= j}, {its row, its orig},
+//                   {last NaN row, its orig}, {best |v| over non-NaN rows
+//                   after the last NaN}, {its row, its orig}   (row -1: none),
+//                   {orig of row j} (workgroup 0), two pad entries
+//   rows[2][P][3][PW]: the best, the last-NaN and the after-NaN row
+//   jrow[2][PW]: the diagonal row's values
+// Every store covers whole 128-byte lines (a wave's lanes write consecutive
+// entries): a partial-line write to uncached memory is a read-modify-write.
 struct PanelComm {
-  double* hdr;      // [2][P][HDR]
-  double* rows;     // [2][P][3][PW]   best row, NaN row, after-NaN row
-  double* jrow;     // [2][PW + 1]     diagonal row values (+ orig)
-  int* seq;         // [2][P]          column (+1) whose slot is published
-  int* err;         // wait-limit flag
+  u64x2* hdr;
+  u64x2* rows;
+  u64x2* jrow;
+  int* err;  // wait-limit flag
 };
 
-__device__ __forceinline__ void st_d(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void put(u64x2* p, unsigned lo, unsigned hi, unsigned tag) {
+  u64x2 w;
+  w.x = (unsigned long long)lo | ((unsigned long long)tag << 32);
+  w.y = (unsigned long long)hi | ((unsigned long long)tag << 32);
+  *(gv_u64x2*)p = w;
 }
-__device__ __forceinline__ double ld_d(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void put_d(u64x2* p, double d, unsigned tag) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  put(p, (unsigned)b, (unsigned)(b >> 32), tag);
 }
+__device__ __forceinline__ u64x2 get(const u64x2* p) {
+  return *(gv_u64x2*)p;
+}
+__device__ __forceinline__ bool tagged(u64x2 w, unsigned tag) {
+  return (unsigned)(w.x >> 32) == tag && (unsigned)(w.y >> 32) == tag;
+}
+__device__ __forceinline__ double as_d(u64x2 w) {
+  return __longlong_as_double((long long)((w.x & 0xffffffffull) | (w.y << 32)));
+}
+__device__ __forceinline__ int lo_i(u64x2 w) { return (int)(unsigned)w.x; }
+__device__ __forceinline__ int hi_i(u64x2 w) { return (int)(unsigned)w.y; }
 
-// First-max combine of (value, row): larger value wins, ties -> smaller row.
-__device__ __forceinline__ void better(double& bv, int& br, int& bo, double v, int r, int o) {
-  if (r < 0) return;
-  if (br < 0 || v > bv || (v == bv && r < br)) {
-    bv = v;
-    br = r;
-    bo = o;
+// Register-array access at a wave-uniform index: a scalar branch tree picks
+// the register (no per-element selects).
+#define FLA_C31(X)                                                                            \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)
+__device__ __forceinline__ double get32(const double (&a)[32], int i) {
+  double x = a[31];
+  switch (i) {
+#define FLA_G(k) \
+  case k:        \
+    x = a[k];    \
+    break;
+    FLA_C31(FLA_G)
+#undef FLA_G
+    default:
+      break;
+  }
+  return x;
+}
+__device__ __forceinline__ void set32(double (&a)[32], int i, double x) {
+  switch (i) {
+#define FLA_S(k) \
+  case k:        \
+    a[k] = x;    \
+    break;
+    FLA_C31(FLA_S)
+#undef FLA_S
+    default:
+      a[31] = x;
+      break;
+  }
+}
+// a[c] -= l * u[c] for c in (i, 32), i wave-uniform (i = -1: every c): in
+// chunks of 8 columns, whole chunks without selects, the chunk holding i
+// with them.
+__device__ __forceinline__ void axpy_after(double (&a)[32], int i, double l, const double* u) {
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    if (8 * cc + 7 <= i) continue;
+    double uu[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) uu[c] = u[8 * cc + c];
+    if (8 * cc > i) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) a[8 * cc + c] = fma(-l, uu[c], a[8 * cc + c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double x = fma(-l, uu[c], a[8 * cc + c]);
+        a[8 * cc + c] = (8 * cc + c > i) ? x : a[8 * cc + c];
+      }
+    }
   }
 }
 
-__device__ __forceinline__ double pick(const double (&row)[32], int cj) {
-  double x = 0.0;
+// First maximum over the wave of key[i] (the lane's rows lane + 64 i, i =
+// 0..3, so smaller i and then smaller lane means the smaller row); keys below
+// zero take no part.  Returns the local row, or -1, and the maximum.
+__device__ __forceinline__ int first_max(const double (&key)[4], double& best) {
+  double m = fmax(fmax(key[0], key[1]), fmax(key[2], key[3]));
 #pragma unroll
-  for (int c = 0; c < 32; ++c)
-    if (c == cj) x = row[c];
-  return x;
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  best = m;
+  if (m < 0.0) return -1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const unsigned long long b = __ballot(key[i] == m);
+    if (b) return 64 * i + __builtin_ctzll(b);
+  }
+  return -1;
 }
 
 __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __restrict__ A,
                                                       long lda, int k, int* __restrict__ piv,
                                                       int* __restrict__ info,
-                                                      int* __restrict__ moved, PanelComm cm) {
+                                                      int* __restrict__ moved, PanelComm cm,
+                                                      unsigned tagbase) {
   // A wait that expired in an earlier panel of this factorization aborts all later ones.
   if (__hip_atomic_load(cm.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-  const unsigned P = gridDim.x;
+  const int P = gridDim.x;
   const int wg = blockIdx.x;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
-  const int cg = wave & 3;   // column group
-  const int rh = wave >> 2;  // row half
+  const int cg = wave >> 1;  // column group (its two waves sit on different SIMDs)
+  const int rh = wave & 1;   // row half
   const int base = wg * ROWS_PER_WG;
 
-  __shared__ double prow[PW];   // pivot row values
-  __shared__ double jrow[PW];   // old diagonal row values
-  __shared__ double cand[3][PW];
-  __shared__ int jorig;
-  __shared__ double lv[ROWS_PER_WG];
-  __shared__ double wred[2][8];
-  __shared__ double hsum[8];
-  __shared__ int sel[4];        // winner row, winner orig, orig of the old diagonal row
+  __shared__ double colbuf[2][ROWS_PER_WG];  // column j of every row, before its interchange
+  __shared__ int origs[ROWS_PER_WG];         // original row of each local row
+  __shared__ double prow[PW];                // pivot row values
+  __shared__ double jrow[PW];                // old diagonal row values
+  __shared__ int sel[4];                     // pivot row, its orig, orig of the old row j
+  __shared__ double stage[NT / 64][32];      // a row's 32 columns, per wave, to store coalesced
 
-  // Thread owns rows base + rh*128 + 64*q + lane (q = 0..1), columns 32*cg + c.
+  if (wg == 0 && t == 0) {
+    // The move list is appended to only after every workgroup has read this
+    // workgroup's first header, which is stored after this reset completes.
+    __hip_atomic_store(&moved[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+
+  // Thread owns local rows rh*128 + 64*q + lane (q = 0..1), columns 32*cg + c.
   double v[RQ][32];
-  int orig[RQ];
 #pragma unroll
   for (int q = 0; q < RQ; ++q) {
     const int r = base + rh * 128 + 64 * q + lane;
-    orig[q] = r;
 #pragma unroll
     for (int c = 0; c < 32; ++c) {
       const int col = 32 * cg + c;
       v[q][c] = (r < M && col < W) ? A[(long)r + (long)col * lda] : 0.0;
+    }
+    if (cg == 0) {
+      colbuf[0][rh * 128 + 64 * q + lane] = v[q][0];
+      origs[rh * 128 + 64 * q + lane] = r;
     }
   }
 
   for (int j = 0; j < W; ++j) {
     const int par = j & 1;
     const int cgj = j >> 5, cj = j & 31;
-    double* hdr = cm.hdr + ((long)par * P + wg) * HDR;
-    double* rows = cm.rows + ((long)par * P + wg) * 3 * PW;
-    double* jslot = cm.jrow + (long)par * (PW + 1);
+    const unsigned tag = tagbase | (unsigned)(j + 1);
+    __syncthreads();  // column j and the row origins are in LDS
 
-    // ---- local candidates: the two waves holding column j ----
-    if (cg == cgj) {
-      double bv = 0.0, av = 0.0;
-      int br = -1, bo = -1, ar = -1, ao = -1, nr = -1, no = -1;
+    // ---- local candidates, found by every wave from the LDS copy ----
+    double key[4];
+    unsigned long long nanb[4];
 #pragma unroll
-      for (int q = 0; q < RQ; ++q) {
-        const int r = base + rh * 128 + 64 * q + lane;
-        const double x = pick(v[q], cj);
-        if (r < M && r >= j) {
-          if (x != x) {
-            if (r > nr) {
-              nr = r;
-              no = orig[q];
-            }
+    for (int i = 0; i < 4; ++i) {
+      const int lr = lane + 64 * i;
+      const int r = base + lr;
+      const double x = colbuf[par][lr];
+      const bool act = r >= j && r < M;
+      key[i] = (act && x == x) ? fabs(x) : -1.0;
+      nanb[i] = __ballot(act && x != x);
+    }
+    double bv, av = -1.0;
+    const int lb = first_max(key, bv);
+    int ln = -1;
+#pragma unroll
+    for (int i = 3; i >= 0; --i)
+      if (ln < 0 && nanb[i]) ln = 64 * i + 63 - __builtin_clzll(nanb[i]);
+    int la = -1;
+    if (ln >= 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (lane + 64 * i <= ln) key[i] = -1.0;
+      la = first_max(key, av);
+    }
+
+    // ---- publish: the header (wave 0), each candidate row (its owners) ----
+    u64x2* myrows = cm.rows + ((long)par * P + wg) * 3 * PW;
+    u64x2* js = cm.jrow + (long)par * PW;
+    if (wave == 0 && lane < HW) {
+      u64x2* h = cm.hdr + ((long)par * P + wg) * HW;
+      const int br = lb >= 0 ? base + lb : -1, bo = lb >= 0 ? origs[lb] : -1;
+      const int nr = ln >= 0 ? base + ln : -1, no = ln >= 0 ? origs[ln] : -1;
+      const int ar = la >= 0 ? base + la : -1, ao = la >= 0 ? origs[la] : -1;
+      const int jo = (wg == 0) ? origs[j] : -1;
+      const double d = lane == 0 ? bv : av;
+      const unsigned long long db = (unsigned long long)__double_as_longlong(d);
+      unsigned lo = 0, hi = 0;
+      if (lane == 0 || lane == 3) {
+        lo = (unsigned)db;
+        hi = (unsigned)(db >> 32);
+      } else if (lane == 1) {
+        lo = (unsigned)br;
+        hi = (unsigned)bo;
+      } else if (lane == 2) {
+        lo = (unsigned)nr;
+        hi = (unsigned)no;
+      } else if (lane == 4) {
+        lo = (unsigned)ar;
+        hi = (unsigned)ao;
+      } else if (lane == 5) {
+        lo = (unsigned)jo;
+      }
+      put(h + lane, lo, hi, tag);
+    }
+    {
+      // rows lb, ln, la of this workgroup, and row j in workgroup 0: the
+      // owner lane stages its 32 columns, then 32 lanes store them
+      const int jl = (wg == 0) ? j : -1;
+      const int want[4] = {lb, ln, la, jl};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int lr = want[s];
+        if (lr < 0 || (lr >> 7) != rh) continue;  // uniform
+        u64x2* dst = (s < 3) ? myrows + s * PW : js;
+        if (lane == (lr & 63)) {
+          if (((lr >> 6) & 1) == 0) {
+#pragma unroll
+            for (int c = 0; c < 32; ++c) stage[wave][c] = v[0][c];
           } else {
-            better(bv, br, bo, fabs(x), r, orig[q]);
+#pragma unroll
+            for (int c = 0; c < 32; ++c) stage[wave][c] = v[1][c];
           }
         }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int orr = __shfl_xor(br, off), oo = __shfl_xor(bo, off);
-        better(bv, br, bo, ov, orr, oo);
-        const int onr = __shfl_xor(nr, off), ono = __shfl_xor(no, off);
-        if (onr > nr) {
-          nr = onr;
-          no = ono;
-        }
-      }
-      if (lane == 0) {
-        wred[rh][0] = bv;
-        wred[rh][1] = br;
-        wred[rh][2] = bo;
-        wred[rh][3] = nr;
-        wred[rh][4] = no;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 32 && 32 * cg + lane < W) put_d(&dst[32 * cg + lane], stage[wave][lane], tag);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
-    __syncthreads();
-    if (t == 0) {
-      double bv = wred[0][0];
-      int br = (int)wred[0][1], bo = (int)wred[0][2];
-      better(bv, br, bo, wred[1][0], (int)wred[1][1], (int)wred[1][2]);
-      int nr = (int)wred[0][3], no = (int)wred[0][4];
-      if ((int)wred[1][3] > nr) {
-        nr = (int)wred[1][3];
-        no = (int)wred[1][4];
-      }
-      hsum[0] = bv;
-      hsum[1] = br;
-      hsum[2] = bo;
-      hsum[3] = nr;
-      hsum[4] = no;
-    }
-    __syncthreads();
-    // after-NaN candidate: first max over non-NaN rows > last NaN of this slab
-    const int nr_wg = (int)hsum[3];
-    if (nr_wg >= 0) {
-      if (cg == cgj) {
-        double av = 0.0;
-        int ar = -1, ao = -1;
-#pragma unroll
-        for (int q = 0; q < RQ; ++q) {
-          const int r = base + rh * 128 + 64 * q + lane;
-          const double x = pick(v[q], cj);
-          if (r < M && r > nr_wg && x == x) better(av, ar, ao, fabs(x), r, orig[q]);
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(av, off);
-          const int orr = __shfl_xor(ar, off), oo = __shfl_xor(ao, off);
-          better(av, ar, ao, ov, orr, oo);
-        }
-        if (lane == 0) {
-          wred[rh][5] = av;
-          wred[rh][6] = ar;
-          wred[rh][7] = ao;
-        }
-      }
-      __syncthreads();
-      if (t == 0) {
-        double av = wred[0][5];
-        int ar = (int)wred[0][6], ao = (int)wred[0][7];
-        better(av, ar, ao, wred[1][5], (int)wred[1][6], (int)wred[1][7]);
-        hsum[5] = av;
-        hsum[6] = ar;
-        hsum[7] = ao;
-      }
-    } else if (t == 0) {
-      hsum[5] = 0.0;
-      hsum[6] = -1.0;
-      hsum[7] = -1.0;
-    }
-    __syncthreads();
-    const int br = (int)hsum[1], nr = (int)hsum[3], ar = (int)hsum[6];
-    // Stage candidate rows (and the diagonal row j) in LDS: one branch per
-    // row around its 32 stores; waves holding none skip (a uniform test).
-#pragma unroll
-    for (int q = 0; q < RQ; ++q) {
-      const int r = base + rh * 128 + 64 * q + lane;
-      if (!__any(r == br || r == nr || r == ar || r == j)) continue;
-      if (r == br) {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) cand[0][32 * cg + c] = v[q][c];
-      }
-      if (r == nr) {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) cand[1][32 * cg + c] = v[q][c];
-      }
-      if (r == ar) {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) cand[2][32 * cg + c] = v[q][c];
-      }
-      if (r == j) {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) jrow[32 * cg + c] = v[q][c];
-        if (cg == 0) jorig = orig[q];
-      }
-    }
-    __syncthreads();
-    // ---- wave 0: publish this workgroup's slot for column j, wait for every
-    // slot of column j, then pick the global pivot ----
+
+    // ---- wave 0: read every header, pick the pivot, read its row and row j ----
     if (wave == 0) {
-      const bool owns_j = (j >= base && j < base + ROWS_PER_WG);
-      if (lane < HDR) st_d(&hdr[lane], hsum[lane]);
-      for (int e = lane; e < W; e += 64) {
-        if (br >= 0) st_d(&rows[0 * PW + e], cand[0][e]);
-        if (nr >= 0) st_d(&rows[1 * PW + e], cand[1][e]);
-        if (ar >= 0) st_d(&rows[2 * PW + e], cand[2][e]);
-        if (owns_j) st_d(&jslot[e], jrow[e]);
-      }
-      if (owns_j && lane == 0) st_d(&jslot[PW], (double)jorig);
-      // The exchange buffers are uncached: waiting for the stores to be
-      // acknowledged orders them before the sequence number, with no L2
-      // write-back.
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      if (lane == 0)
-        __hip_atomic_store(&cm.seq[par * P + wg], j + 1, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
       bool ok = true;
       const unsigned long long t0 = wall_clock64();
-      while (true) {
-        bool done = true;
-        for (unsigned w = lane; w < P; w += 64)
-          if (__hip_atomic_load(&cm.seq[par * P + w], __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT) != j + 1)
-            done = false;
-        if (__all(done)) break;
+      auto expired = [&]() {
         if (__hip_atomic_load(cm.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
             wall_clock64() - t0 > kSpinTicks) {
           if (lane == 0) __hip_atomic_store(cm.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = false;
-          break;
+          return true;
         }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // uncached: no invalidate
-      if (!ok) {
-        if (lane == 0) sel[0] = -2;
-      } else {
-        double gnr = -1.0, gno = -1.0;
-        // the first slot's no-NaN candidate comes in the same round trip
-        double f0 = 0.0, f1 = -1.0, f2 = -1.0;
-        for (unsigned w = lane; w < P; w += 64) {
-          const double* h = cm.hdr + ((long)par * P + w) * HDR;
-          const double hn = ld_d(&h[3]), ho = ld_d(&h[4]);
-          const double h0 = ld_d(&h[0]), h1 = ld_d(&h[1]), h2 = ld_d(&h[2]);
-          if (w == lane) {
-            f0 = h0;
-            f1 = h1;
-            f2 = h2;
+        return false;
+      };
+      const u64x2* hb = cm.hdr + (long)par * P * HW;
+      // pass 1: best of every slab, and the last NaN row overall
+      double gv = 0.0;
+      int grow = -1, go = -1, gw = -1, gslot = 0, gnr = -1, gno = -1, jo = 0;
+      for (int w0 = 0; ok && w0 < P; w0 += 64) {
+        const int w = w0 + lane;
+        u64x2 e0, e1, e2, e3, e4, e5;
+        while (true) {
+          bool done = true;
+          if (w < P) {
+            const u64x2* h = hb + (long)w * HW;
+            e0 = get(h);
+            e1 = get(h + 1);
+            e2 = get(h + 2);
+            e3 = get(h + 3);
+            e4 = get(h + 4);
+            e5 = get(h + 5);
+            done = tagged(e0, tag) && tagged(e1, tag) && tagged(e2, tag) && tagged(e3, tag) &&
+                   tagged(e4, tag) && tagged(e5, tag);
           }
+          if (__all(done)) break;
+          if (expired()) {
+            ok = false;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (ok && w == 0) jo = lo_i(e5);
+        if (ok && w < P) {
+          const double cv = as_d(e0);
+          const int cr = lo_i(e1), co = hi_i(e1), hn = lo_i(e2), hno = hi_i(e2);
           if (hn > gnr) {
             gnr = hn;
-            gno = ho;
+            gno = hno;
+          }
+          if (cr >= 0 && (grow < 0 || cv > gv || (cv == gv && cr < grow))) {
+            gv = cv;
+            grow = cr;
+            go = co;
+            gw = w;
           }
         }
+      }
+      if (ok) {
+#pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-          const double o = __shfl_xor(gnr, off), oo = __shfl_xor(gno, off);
+          const int o = __shfl_xor(gnr, off), oo = __shfl_xor(gno, off);
           if (o > gnr) {
             gnr = o;
             gno = oo;
           }
         }
-        // Candidates: without a NaN, every slab's best; with the last NaN at
-        // row gnr (idamax then restarts after it), the after-NaN candidate of
-        // the slab holding gnr plus the best of every slab wholly below it.
-        double bv = 0.0;
-        int bw = -1, brow = -1, bo = -1, bslot = 0;
-        for (unsigned w = lane; w < P; w += 64) {
-          const double* h = cm.hdr + ((long)par * P + w) * HDR;
-          double cv = 0.0;
-          int cr = -1, co = -1, cs = 0;
-          if (gnr < 0.0) {
-            cv = (w == lane) ? f0 : ld_d(&h[0]);
-            cr = (int)((w == lane) ? f1 : ld_d(&h[1]));
-            co = (int)((w == lane) ? f2 : ld_d(&h[2]));
-          } else {
-            const int hn = (int)ld_d(&h[3]);
-            if (hn == (int)gnr) {
-              cv = ld_d(&h[5]);
-              cr = (int)ld_d(&h[6]);
-              co = (int)ld_d(&h[7]);
+        if (gnr >= 0) {
+          // With the last NaN at row gnr (idamax restarts after it): the
+          // after-NaN candidate of the slab holding gnr plus the best of
+          // every slab wholly below it.  All headers are current by now.
+          gv = 0.0;
+          grow = go = gw = -1;
+          for (int w = lane; w < P; w += 64) {
+            const u64x2* h = hb + (long)w * HW;
+            double cv = 0.0;
+            int cr = -1, co = -1, cs = 0;
+            if (lo_i(get(h + 2)) == gnr) {
+              const u64x2 e3 = get(h + 3), e4 = get(h + 4);
+              cv = as_d(e3);
+              cr = lo_i(e4);
+              co = hi_i(e4);
               cs = 2;
             } else if ((long)w * ROWS_PER_WG > (long)gnr) {
-              cv = ld_d(&h[0]);
-              cr = (int)ld_d(&h[1]);
-              co = (int)ld_d(&h[2]);
+              const u64x2 e0 = get(h), e1 = get(h + 1);
+              cv = as_d(e0);
+              cr = lo_i(e1);
+              co = hi_i(e1);
+            }
+            if (cr >= 0 && (grow < 0 || cv > gv || (cv == gv && cr < grow))) {
+              gv = cv;
+              grow = cr;
+              go = co;
+              gw = w;
+              gslot = cs;
             }
           }
-          if (cr >= 0 && (brow < 0 || cv > bv || (cv == bv && cr < brow))) {
-            bv = cv;
-            brow = cr;
-            bo = co;
-            bw = (int)w;
-            bslot = cs;
-          }
         }
+#pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(bv, off);
-          const int orow = __shfl_xor(brow, off), oo = __shfl_xor(bo, off),
-                    ow = __shfl_xor(bw, off), os = __shfl_xor(bslot, off);
-          if (orow >= 0 && (brow < 0 || ov > bv || (ov == bv && orow < brow))) {
-            bv = ov;
-            brow = orow;
-            bo = oo;
-            bw = ow;
-            bslot = os;
+          const double ov = __shfl_xor(gv, off);
+          const int orow = __shfl_xor(grow, off), oo = __shfl_xor(go, off),
+                    ow = __shfl_xor(gw, off), os = __shfl_xor(gslot, off);
+          if (orow >= 0 && (grow < 0 || ov > gv || (ov == gv && orow < grow))) {
+            gv = ov;
+            grow = orow;
+            go = oo;
+            gw = ow;
+            gslot = os;
           }
         }
-        if (brow < 0 && gnr >= 0.0) {  // nothing after the last NaN: it is the pivot
-          brow = (int)gnr;
-          bo = (int)gno;
-          bw = brow / ROWS_PER_WG;
-          bslot = 1;
+        if (grow < 0 && gnr >= 0) {  // nothing after the last NaN: it is the pivot
+          grow = gnr;
+          go = gno;
+          gw = gnr / ROWS_PER_WG;
+          gslot = 1;
         }
-        // the winning row and the old diagonal row (from its owner's slot)
-        const double* src = cm.rows + (((long)par * P + bw) * 3 + bslot) * PW;
-        for (int e = lane; e < W; e += 64) {
-          prow[e] = ld_d(&src[e]);
-          jrow[e] = ld_d(&jslot[e]);
+        if (gw < 0) gw = 0;  // unreachable: row j itself takes part (W <= M)
+        // the winning row and the old diagonal row
+        const u64x2* src = cm.rows + (((long)par * P + gw) * 3 + gslot) * PW;
+        while (true) {
+          bool done = true;
+          for (int e = lane; e < W; e += 64) {
+            const u64x2 a = get(&src[e]), b = get(&js[e]);
+            if (tagged(a, tag) && tagged(b, tag)) {
+              prow[e] = as_d(a);
+              jrow[e] = as_d(b);
+            } else {
+              done = false;
+            }
+          }
+          if (__all(done)) break;
+          if (expired()) {
+            ok = false;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
         }
         if (lane == 0) {
-          sel[0] = brow;
-          sel[1] = bo;
-          sel[2] = (int)ld_d(&jslot[PW]);
+          sel[0] = grow;
+          sel[1] = go;
+          sel[2] = jo;
         }
       }
+      if (!ok && lane == 0) sel[0] = -2;
     }
     __syncthreads();
     if (sel[0] == -2) return;
     const int r = sel[0];
-    const int origj = sel[2];
-
     const double piv_val = prow[j];
     const bool null_pivot = (piv_val == 0.0);
+    const bool swap = !null_pivot && r != j;
     if (wg == 0 && t == 0) {
       piv[k + j] = r - j;
       if (null_pivot) atomicCAS(info, -1, k + j);
     }
     // ---- interchange rows j and r (skipped for a null pivot) ----
-    if (!null_pivot && r != j) {
+    if (swap) {
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
         const int rr = base + rh * 128 + 64 * q + lane;
@@ -384,69 +449,55 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         if (rr == j) {
 #pragma unroll
           for (int c = 0; c < 32; ++c) v[q][c] = prow[32 * cg + c];
-          orig[q] = sel[1];
         } else if (rr == r) {
 #pragma unroll
           for (int c = 0; c < 32; ++c) v[q][c] = jrow[32 * cg + c];
-          orig[q] = origj;
         }
       }
+      if (t == 0) {
+        if (j >= base && j < base + ROWS_PER_WG) origs[j - base] = sel[1];
+        if (r >= base && r < base + ROWS_PER_WG) origs[r - base] = sel[2];
+      }
     }
-    // ---- scale column j below the diagonal (by the reciprocal, skipped when
-    // the pivot is 1, as bl1_dinvscalv), then the rank-1 update ----
+    // ---- multipliers (column j scaled by the reciprocal, skipped for a null
+    // pivot or a pivot of 1, as bl1_dinvscalv), then the rank-1 update with
+    // the diagonal row (the old row j when there was no interchange) ----
     const bool scale = !null_pivot && piv_val != 1.0;
     const double inv = scale ? 1.0 / piv_val : 1.0;
-    if (cg == cgj) {
+    const double* u = swap ? prow : jrow;
+    const int jn = j + 1;
 #pragma unroll
-      for (int q = 0; q < RQ; ++q) {
-        const int lr = rh * 128 + 64 * q + lane;
-        const int rr = base + lr;
-        double x = pick(v[q], cj);
-        if (rr > j && rr < M) {
-          if (scale) x = x * inv;
-#pragma unroll
-          for (int c = 0; c < 32; ++c)
-            if (c == cj) v[q][c] = x;
-        } else {
-          x = 0.0;
-        }
-        lv[lr] = x;
+    for (int q = 0; q < RQ; ++q) {
+      const int lr = rh * 128 + 64 * q + lane;
+      const int rr = base + lr;
+      const bool act = rr > j && rr < M;
+      double x = colbuf[par][lr];
+      if (swap && rr == r) x = jrow[j];
+      const double l = scale ? x * inv : x;
+      if (act && cg >= cgj) {
+        if (cg == cgj) set32(v[q], cj, l);
+        axpy_after(v[q], cg == cgj ? cj : -1, l, u + 32 * cg);
       }
+      if (jn < W && cg == (jn >> 5)) colbuf[par ^ 1][lr] = get32(v[q], jn & 31);
     }
-    __syncthreads();
-    if (32 * cg + 31 > j) {
-      // Branch-free (results selected), so the prow loads are batched.
-#pragma unroll
-      for (int q = 0; q < RQ; ++q) {
-        const int lr = rh * 128 + 64 * q + lane;
-        const int rr = base + lr;
-        const bool act = rr > j && rr < M;
-        const double l = lv[lr];
-#pragma unroll
-        for (int c = 0; c < 32; ++c) {
-          const int col = 32 * cg + c;
-          const double u = v[q][c] - l * prow[col];
-          v[q][c] = (act && col > j && col < W) ? u : v[q][c];
-        }
-      }
-    }
-    __syncthreads();
   }
+  __syncthreads();
 
   // ---- write back; record rows whose content moved ----
 #pragma unroll
   for (int q = 0; q < RQ; ++q) {
-    const int r = base + rh * 128 + 64 * q + lane;
+    const int lr = rh * 128 + 64 * q + lane;
+    const int r = base + lr;
     if (r < M) {
 #pragma unroll
       for (int c = 0; c < 32; ++c) {
         const int col = 32 * cg + c;
         if (col < W) A[(long)r + (long)col * lda] = v[q][c];
       }
-      if (cg == 0 && orig[q] != r) {
+      if (cg == 0 && origs[lr] != r) {
         const int slot = atomicAdd(&moved[0], 1);
         moved[1 + 2 * slot] = r;
-        moved[2 + 2 * slot] = orig[q];
+        moved[2 + 2 * slot] = origs[lr];
       }
     }
   }
@@ -500,16 +551,14 @@ __global__ __launch_bounds__(256) void transpose_kernel(int r, int c, const doub
   }
 }
 
-__global__ void lu_reset_kernel(int* seq, int nseq, int* moved) {
-  for (int i = threadIdx.x; i < nseq; i += blockDim.x) seq[i] = 0;
-  if (threadIdx.x == 0) moved[0] = 0;
-}
-
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 }  // namespace
 
-// Buffers for one panel launch (two sets, alternated between steps).
+// Exchange area of the panel kernel (one, shared by every launch: the tags
+// keep launches apart) and two move lists, alternated between steps, since
+// the previous step's row gathers may still read theirs while the next panel
+// runs.
 struct LuBufs {
   PanelComm cm;
   int* moved;
@@ -517,35 +566,54 @@ struct LuBufs {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static size_t lu_set_bytes(int Pmax) {
-  return align256(sizeof(double) * 2 * Pmax * HDR) + align256(sizeof(double) * 2 * Pmax * 3 * PW) +
-         align256(sizeof(double) * 2 * (PW + 1)) + align256(sizeof(int) * 2 * Pmax) +
-         align256(sizeof(int)) + align256(sizeof(int) * (1 + 4 * PW));
+constexpr size_t kHdrBytes = sizeof(u64x2) * 2 * MAX_P * HW;
+constexpr size_t kRowsBytes = sizeof(u64x2) * 2 * MAX_P * 3 * PW;
+constexpr size_t kJrowBytes = sizeof(u64x2) * 2 * PW;
+constexpr size_t kMovedBytes = sizeof(int) * (1 + 4 * PW);
+static_assert(kHdrBytes + kRowsBytes + kJrowBytes + 256 * 8 + 2 * kMovedBytes < kUcLuBytes,
+              "LU exchange area");
+
+static void lu_carve(char* p, LuBufs (&b)[2]) {
+  PanelComm cm;
+  cm.hdr = reinterpret_cast<u64x2*>(p);
+  p += align256(kHdrBytes);
+  cm.rows = reinterpret_cast<u64x2*>(p);
+  p += align256(kRowsBytes);
+  cm.jrow = reinterpret_cast<u64x2*>(p);
+  p += align256(kJrowBytes);
+  cm.err = nullptr;  // set by the caller
+  for (int s = 0; s < 2; ++s) {
+    b[s].cm = cm;
+    b[s].moved = reinterpret_cast<int*>(p);
+    p += align256(kMovedBytes);
+  }
 }
 
-static LuBufs lu_carve(char* p, int Pmax) {
-  LuBufs b;
-  b.cm.hdr = reinterpret_cast<double*>(p);
-  p += align256(sizeof(double) * 2 * Pmax * HDR);
-  b.cm.rows = reinterpret_cast<double*>(p);
-  p += align256(sizeof(double) * 2 * Pmax * 3 * PW);
-  b.cm.jrow = reinterpret_cast<double*>(p);
-  p += align256(sizeof(double) * 2 * (PW + 1));
-  b.cm.seq = reinterpret_cast<int*>(p);
-  p += align256(sizeof(int) * 2 * Pmax);
-  b.cm.err = reinterpret_cast<int*>(p);
-  p += align256(sizeof(int));
-  b.moved = reinterpret_cast<int*>(p);
-  return b;
+static __global__ void zero_kernel(unsigned long long* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
+
+// Launch counter: its low 24 bits tag every exchange word of a launch.  When
+// it wraps, the exchange area is cleared, so no word written 2^24 launches
+// earlier can carry a current tag.
+static unsigned g_lu_epoch = 0;
 
 static int lu_panel(hipStream_t s, int M, int W, double* Ap, long lda, int k, int* piv,
                     int* dinfo, const LuBufs& b) {
-  hipLaunchKernelGGL(lu_reset_kernel, dim3(1), dim3(256), 0, s, b.cm.seq, 2 * ((M + ROWS_PER_WG - 1) / ROWS_PER_WG), b.moved);
   const int P = (M + ROWS_PER_WG - 1) / ROWS_PER_WG;
+  g_lu_epoch = (g_lu_epoch + 1) & 0xffffffu;
+  if (g_lu_epoch == 0) {
+    const long words = (long)(reinterpret_cast<char*>(b.cm.jrow) + kJrowBytes -
+                              reinterpret_cast<char*>(b.cm.hdr)) / 8;
+    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(b.cm.hdr), words);
+    FLA_HIP_CHECK(hipGetLastError());
+    g_lu_epoch = 1;
+  }
   void* tok = fla_stats_begin(s);
   hipLaunchKernelGGL(lu_panel_kernel, dim3(P), dim3(NT), 0, s, M, W, Ap, lda, k, piv, dinfo,
-                     b.moved, b.cm);
+                     b.moved, b.cm, g_lu_epoch << 8);
   double fl = 0.0;
   for (int j = 0; j < W; ++j) fl += (double)(M - j - 1) * (1.0 + 2.0 * (W - j - 1));
   fla_stats_end(s, tok, 2, fl);
@@ -601,11 +669,9 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
                  c.num_cus * ROWS_PER_WG);
     return FLA_HIP_EARG;
   }
-  // exchange buffers in the uncached region, laid out for MAX_P workgroups
-  const size_t set = align256(lu_set_bytes(MAX_P));
-  static_assert(2 * (8 * 2 * MAX_P * HDR + 8 * 2 * MAX_P * 3 * PW) < kUcLuBytes, "LU exchange");
-  char* sp = c.uc;
-  LuBufs bufs[2] = {lu_carve(sp, MAX_P), lu_carve(sp + set, MAX_P)};
+  // exchange area in the uncached region, laid out for MAX_P workgroups
+  LuBufs bufs[2];
+  lu_carve(c.uc, bufs);
   // transposed U12 of one step (n x nb, ld n)
   void* wut = nullptr;
   FLA_TRY(fla_workspace((size_t)(n > 1 ? n : 1) * PW * sizeof(double), &wut));
@@ -678,9 +744,7 @@ int fla_lu_panel_setup(int m, int* derr) {
                  c.num_cus * ROWS_PER_WG);
     return FLA_HIP_EARG;
   }
-  const size_t set = align256(lu_set_bytes(MAX_P));
-  g_mp_bufs[0] = lu_carve(c.uc, MAX_P);
-  g_mp_bufs[1] = lu_carve(c.uc + set, MAX_P);
+  lu_carve(c.uc, g_mp_bufs);
   g_mp_bufs[0].cm.err = derr;
   g_mp_bufs[1].cm.err = derr;
   return FLA_HIP_OK;

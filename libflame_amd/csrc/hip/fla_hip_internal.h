@@ -48,8 +48,8 @@ struct FlaHipCtx {
   // that workgroups exchange inside one kernel: stores and loads of it need
   // no L2 write-back or invalidate across the XCDs.  Allocated once and kept
   // until the process ends (this kind of memory must not be freed after a
-  // kernel read it).  [0, kUcLuBytes): the LU panel's two exchange buffer
-  // sets; then the triangular solve's copy of the finished right-hand-side
+  // kernel read it).  [0, kUcLuBytes): the LU panel's exchange area and
+  // move lists; then the triangular solve's copy of the finished right-hand-side
   // blocks (16 columns of n <= kUcSolveRows rows).
   char* uc = nullptr;
   // 128 x 128 FP64 block: inverse of the current Cholesky diagonal block.

@@ -79,9 +79,11 @@ extern "C" int fla_hip_init(void) {
   FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dlinv), 128 * 128 * sizeof(double)));
   // uncached exchange memory: allocated once per process, never freed
   static char* uc = nullptr;
-  if (!uc)
+  if (!uc) {
     FLA_HIP_CHECK(hipExtMallocWithFlags(reinterpret_cast<void**>(&uc), kUcBytes,
                                         hipDeviceMallocUncached));
+    FLA_HIP_CHECK(hipMemset(uc, 0, kUcBytes));  // no exchange word carries a valid tag
+  }
   g_ctx.uc = uc;
   g_ctx.ready = true;
   return FLA_HIP_OK;
