@@ -39,7 +39,8 @@ constexpr int MAX_P = 256;  // panel workgroups (one per CU at most)
 constexpr int HW = 8;       // header entries per workgroup slot (one 128-byte line)
 
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(1))) volatile u64x2 gv_u64x2;  // global, volatile
+typedef __attribute__((address_space(1))) u64x2 g_u64x2;
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
 
 // Exchange area (tagged 16-byte entries, two parities: column j uses j & 1).
 //   hdr [2][P][HW]: {best |v| over non-NaN rows >= j}, {its row, its orig},
@@ -57,18 +58,25 @@ struct PanelComm {
   int* err;  // wait-limit flag
 };
 
+// Stores need no ordering among themselves (each word carries its tag): a
+// plain 16-byte store.  Loads are relaxed atomics of the two 8-byte words, so
+// a polling loop re-reads memory without waiting after every load.
 __device__ __forceinline__ void put(u64x2* p, unsigned lo, unsigned hi, unsigned tag) {
   u64x2 w;
   w.x = (unsigned long long)lo | ((unsigned long long)tag << 32);
   w.y = (unsigned long long)hi | ((unsigned long long)tag << 32);
-  *(gv_u64x2*)p = w;
+  *(g_u64x2*)p = w;
 }
 __device__ __forceinline__ void put_d(u64x2* p, double d, unsigned tag) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(d);
   put(p, (unsigned)b, (unsigned)(b >> 32), tag);
 }
 __device__ __forceinline__ u64x2 get(const u64x2* p) {
-  return *(gv_u64x2*)p;
+  g_u64* q = (g_u64*)p;
+  u64x2 w;
+  w.x = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  w.y = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return w;
 }
 __device__ __forceinline__ bool tagged(u64x2 w, unsigned tag) {
   return (unsigned)(w.x >> 32) == tag && (unsigned)(w.y >> 32) == tag;
@@ -79,68 +87,57 @@ __device__ __forceinline__ double as_d(u64x2 w) {
 __device__ __forceinline__ int lo_i(u64x2 w) { return (int)(unsigned)w.x; }
 __device__ __forceinline__ int hi_i(u64x2 w) { return (int)(unsigned)w.y; }
 
-// Register-array access at a wave-uniform index: a scalar branch tree picks
-// the register (no per-element selects).
-#define FLA_C31(X)                                                                            \
-  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
-  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)
-__device__ __forceinline__ double get32(const double (&a)[32], int i) {
-  double x = a[31];
-  switch (i) {
-#define FLA_G(k) \
-  case k:        \
-    x = a[k];    \
-    break;
-    FLA_C31(FLA_G)
-#undef FLA_G
-    default:
-      break;
-  }
-  return x;
+// Wave reductions with DPP: prefix within each row of 16 lanes (row_shr 1,
+// 2, 4, 8), then across rows (row_bcast 15, 31); lane 63 ends with the
+// result.  Lanes whose source is out of range keep their own value.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, ROWM, 0xf, false);
 }
-__device__ __forceinline__ void set32(double (&a)[32], int i, double x) {
-  switch (i) {
-#define FLA_S(k) \
-  case k:        \
-    a[k] = x;    \
-    break;
-    FLA_C31(FLA_S)
-#undef FLA_S
-    default:
-      a[31] = x;
-      break;
-  }
+template <int CTRL, int ROWM>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_i<CTRL, ROWM>((int)b), hi = dpp_i<CTRL, ROWM>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
-// a[c] -= l * u[c] for c in (i, 32), i wave-uniform (i = -1: every c): in
-// chunks of 8 columns, whole chunks without selects, the chunk holding i
-// with them.
-__device__ __forceinline__ void axpy_after(double (&a)[32], int i, double l, const double* u) {
-#pragma unroll
-  for (int cc = 0; cc < 4; ++cc) {
-    if (8 * cc + 7 <= i) continue;
-    double uu[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) uu[c] = u[8 * cc + c];
-    if (8 * cc > i) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) a[8 * cc + c] = fma(-l, uu[c], a[8 * cc + c]);
-    } else {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const double x = fma(-l, uu[c], a[8 * cc + c]);
-        a[8 * cc + c] = (8 * cc + c > i) ? x : a[8 * cc + c];
-      }
-    }
-  }
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_max_d(double x) {
+  x = fmax(x, dpp_d<0x111, 0xf>(x));
+  x = fmax(x, dpp_d<0x112, 0xf>(x));
+  x = fmax(x, dpp_d<0x114, 0xf>(x));
+  x = fmax(x, dpp_d<0x118, 0xf>(x));
+  x = fmax(x, dpp_d<0x142, 0xa>(x));
+  x = fmax(x, dpp_d<0x143, 0xc>(x));
+  return readlane_d(x, 63);
+}
+__device__ __forceinline__ int wave_max_i(int x) {
+  x = max(x, dpp_i<0x111, 0xf>(x));
+  x = max(x, dpp_i<0x112, 0xf>(x));
+  x = max(x, dpp_i<0x114, 0xf>(x));
+  x = max(x, dpp_i<0x118, 0xf>(x));
+  x = max(x, dpp_i<0x142, 0xa>(x));
+  x = max(x, dpp_i<0x143, 0xc>(x));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+__device__ __forceinline__ int wave_min_i(int x) {
+  x = min(x, dpp_i<0x111, 0xf>(x));
+  x = min(x, dpp_i<0x112, 0xf>(x));
+  x = min(x, dpp_i<0x114, 0xf>(x));
+  x = min(x, dpp_i<0x118, 0xf>(x));
+  x = min(x, dpp_i<0x142, 0xa>(x));
+  x = min(x, dpp_i<0x143, 0xc>(x));
+  return __builtin_amdgcn_readlane(x, 63);
 }
 
 // First maximum over the wave of key[i] (the lane's rows lane + 64 i, i =
 // 0..3, so smaller i and then smaller lane means the smaller row); keys below
 // zero take no part.  Returns the local row, or -1, and the maximum.
 __device__ __forceinline__ int first_max(const double (&key)[4], double& best) {
-  double m = fmax(fmax(key[0], key[1]), fmax(key[2], key[3]));
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  const double m = wave_max_d(fmax(fmax(key[0], key[1]), fmax(key[2], key[3])));
   best = m;
   if (m < 0.0) return -1;
 #pragma unroll
@@ -196,6 +193,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
       origs[rh * 128 + 64 * q + lane] = r;
     }
   }
+  // All panel loads complete here, so that inside the column loop no use of
+  // the panel registers waits on memory counters (which also count the
+  // exchange stores in flight).
+  __builtin_amdgcn_s_waitcnt(0);
 
   for (int j = 0; j < W; ++j) {
     const int par = j & 1;
@@ -328,13 +329,14 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         }
         if (ok && w == 0) jo = lo_i(e5);
         if (ok && w < P) {
+          // a lane visits its slabs in row order: keep the first maximum
           const double cv = as_d(e0);
           const int cr = lo_i(e1), co = hi_i(e1), hn = lo_i(e2), hno = hi_i(e2);
           if (hn > gnr) {
             gnr = hn;
             gno = hno;
           }
-          if (cr >= 0 && (grow < 0 || cv > gv || (cv == gv && cr < grow))) {
+          if (cr >= 0 && (grow < 0 || cv > gv)) {
             gv = cv;
             grow = cr;
             go = co;
@@ -343,15 +345,26 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
         }
       }
       if (ok) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const int o = __shfl_xor(gnr, off), oo = __shfl_xor(gno, off);
-          if (o > gnr) {
-            gnr = o;
-            gno = oo;
+        const unsigned long long nan_lanes = __ballot(gnr >= 0);
+        if (nan_lanes == 0) {
+          // first maximum over the lanes: the wave's maximum, then the
+          // smallest row among the lanes that hold it
+          const double m = wave_max_d(grow >= 0 ? gv : -1.0);
+          const unsigned long long b = __ballot(grow >= 0 && gv == m);
+          int L = b ? __builtin_ctzll(b) : 0;
+          if (__builtin_popcountll(b) > 1) {
+            const bool in = (b >> lane) & 1ull;
+            const int mr = wave_min_i(in ? grow : 0x7fffffff);
+            L = __builtin_ctzll(__ballot(in && grow == mr));
           }
-        }
-        if (gnr >= 0) {
+          grow = __builtin_amdgcn_readlane(grow, L);
+          go = __builtin_amdgcn_readlane(go, L);
+          gw = __builtin_amdgcn_readlane(gw, L);
+        } else {
+          // the last NaN row overall, and its orig from the lane holding it
+          const int m = wave_max_i(gnr);
+          gno = __builtin_amdgcn_readlane(gno, __builtin_ctzll(__ballot(gnr == m)));
+          gnr = m;
           // With the last NaN at row gnr (idamax restarts after it): the
           // after-NaN candidate of the slab holding gnr plus the best of
           // every slab wholly below it.  All headers are current by now.
@@ -381,18 +394,18 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
               gslot = cs;
             }
           }
-        }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(gv, off);
-          const int orow = __shfl_xor(grow, off), oo = __shfl_xor(go, off),
-                    ow = __shfl_xor(gw, off), os = __shfl_xor(gslot, off);
-          if (orow >= 0 && (grow < 0 || ov > gv || (ov == gv && orow < grow))) {
-            gv = ov;
-            grow = orow;
-            go = oo;
-            gw = ow;
-            gslot = os;
+          for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(gv, off);
+            const int orow = __shfl_xor(grow, off), oo = __shfl_xor(go, off),
+                      ow = __shfl_xor(gw, off), os = __shfl_xor(gslot, off);
+            if (orow >= 0 && (grow < 0 || ov > gv || (ov == gv && orow < grow))) {
+              gv = ov;
+              grow = orow;
+              go = oo;
+              gw = ow;
+              gslot = os;
+            }
           }
         }
         if (grow < 0 && gnr >= 0) {  // nothing after the last NaN: it is the pivot
@@ -466,19 +479,42 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     const double inv = scale ? 1.0 / piv_val : 1.0;
     const double* u = swap ? prow : jrow;
     const int jn = j + 1;
+    if (cg >= cgj) {
 #pragma unroll
-    for (int q = 0; q < RQ; ++q) {
-      const int lr = rh * 128 + 64 * q + lane;
-      const int rr = base + lr;
-      const bool act = rr > j && rr < M;
-      double x = colbuf[par][lr];
-      if (swap && rr == r) x = jrow[j];
-      const double l = scale ? x * inv : x;
-      if (act && cg >= cgj) {
-        if (cg == cgj) set32(v[q], cj, l);
-        axpy_after(v[q], cg == cgj ? cj : -1, l, u + 32 * cg);
+      for (int q = 0; q < RQ; ++q) {
+        const int lr = rh * 128 + 64 * q + lane;
+        const int rr = base + lr;
+        const bool act = rr > j && rr < M;
+        double x = colbuf[par][lr];
+        if (swap && rr == r) x = jrow[j];
+        const double l = scale ? x * inv : x;
+        double xn = 0.0;
+        // 8-column chunks [c0, c0 + 8): wholly right of j without selects;
+        // the chunk holding j (only in the waves owning it) with them
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+          const int c0 = 32 * cg + 8 * cc;
+          if (c0 + 7 < j) continue;  // uniform
+          double* a = &v[q][8 * cc];
+          if (c0 > j) {
+            if (act) {
+#pragma unroll
+              for (int c = 0; c < 8; ++c) a[c] = fma(-l, u[c0 + c], a[c]);
+            }
+          } else if (act) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+              const double y = fma(-l, u[c0 + c], a[c]);
+              a[c] = (c0 + c > j) ? y : ((c0 + c == j) ? l : a[c]);
+            }
+          }
+          if (jn < W && c0 <= jn && jn < c0 + 8) {  // uniform: the next column's chunk
+#pragma unroll
+            for (int c = 0; c < 8; ++c) xn = (c0 + c == jn) ? a[c] : xn;
+          }
+        }
+        if (jn < W && cg == (jn >> 5)) colbuf[par ^ 1][lr] = xn;
       }
-      if (jn < W && cg == (jn >> 5)) colbuf[par ^ 1][lr] = get32(v[q], jn & 31);
     }
   }
   __syncthreads();
