@@ -738,14 +738,14 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
     hipEvent_t pd = next_event();
     FLA_HIP_CHECK(hipEventRecord(pd, sq));
     FLA_HIP_CHECK(hipStreamWaitEvent(su, pd, 0));
-    // Interchanges on the columns left of the panel.
-    FLA_TRY(lu_gather(su, k, A + (long)k, lda, b.moved));
     const int k2 = k + kb;
     const int kb2 = (mn - k2) < nb ? (mn - k2) : nb;  // next panel width (0 at the end)
     // (a) the next panel's columns, so that panel can start
     FLA_TRY(lu_update_cols(su, m, A, lda, k, kb, k2, kb2 > 0 ? kb2 : 0, b.moved, Ut, ldu));
     hipEvent_t ready = next_event();
     FLA_HIP_CHECK(hipEventRecord(ready, su));
+    // Interchanges on the columns left of the panel (off the next panel's path).
+    FLA_TRY(lu_gather(su, k, A + (long)k, lda, b.moved));
     // (b) the remaining columns
     FLA_TRY(lu_update_cols(su, m, A, lda, k, kb, k2 + kb2, n - k2 - kb2, b.moved, Ut + kb2, ldu));
     if (kb2 <= 0) break;

@@ -11,8 +11,10 @@
 // of the block it writes).
 //
 // LU panel: A12 := inv(tril_unit(L11)) * A12 (FLA_LU_piv_blk_var5.c:88-91,
-// dtrsm_('L','L','N','U')) runs as one lane per column with the 128-entry
-// column in VGPRs and L11 read through uniform loads.
+// dtrsm_('L','L','N','U')): for a full 128-column panel, a blocked
+// substitution on MFMA (one wave per 16 columns); for a narrower last panel,
+// one lane per column with the column in VGPRs and L11 read through uniform
+// loads.
 #include "fla_hip_internal.h"
 #include "fla_hip_lds_tri.h"
 
@@ -98,6 +100,76 @@ __global__ __launch_bounds__(64) void trsm_llnu_kernel(int ncols, int nb,
     if (FULL || i < nb) col[i] = x[i];
 }
 
+// A12 := inv(L11) * A12 for L11 unit lower of order 128 (ld ldl), in place:
+// one wave per 16 columns, 16-row blocks,
+//   X_i = inv(L_ii) (B_i - sum_{k<i} L_ik X_k),
+// both products on v_mfma_f64_16x16x4f64 with L as the first operand.  The
+// product D(m, n) lands in lane 16*(m%4) + n, register m/4, which is also the
+// second operand's layout for contraction rows 4s + lane/16 in register s, so
+// a finished block feeds the next products straight from its registers.  The
+// eight inv(L_ii) (16 x 16, unit lower) are formed in LDS by every workgroup.
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void trsm_llnu128_kernel(int ncols,
+                                                           const double* __restrict__ L, long ldl,
+                                                           double* __restrict__ B, long ldb) {
+  __shared__ double Ld[8][16][17];  // strictly lower parts of the diagonal blocks
+  __shared__ double Li[8][16][17];  // their unit lower inverses
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int e = t; e < 8 * 256; e += 256) {
+    const int i = e >> 8, c = (e >> 4) & 15, r = e & 15;
+    Ld[i][r][c] = (r > c) ? L[(long)(16 * i + r) + (long)(16 * i + c) * ldl] : 0.0;
+  }
+  __syncthreads();
+  if (t < 128) {  // column c of inv(L_ii) by forward substitution
+    const int i = t >> 4, c = t & 15;
+    double y[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < r; ++q) acc = fma(Ld[i][r][q], y[q], acc);
+      y[r] = (r < c) ? 0.0 : ((r == c) ? 1.0 : -acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) Li[i][r][c] = y[r];
+  }
+  __syncthreads();
+  const int c0 = (blockIdx.x * 4 + wave) * 16;
+  if (c0 >= ncols) return;
+  const int ln = lane & 15, lq = lane >> 4;
+  const int col = c0 + ln;
+  const bool cv = col < ncols;
+  double* bcol = B + (long)col * ldb;
+  v4d_t x[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[i][r] = cv ? bcol[16 * i + 4 * r + lq] : 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    v4d_t acc = x[i];
+#pragma unroll
+    for (int kb = 0; kb < i; ++kb) {
+      double p[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) p[s] = -L[(long)(16 * i + ln) + (long)(16 * kb + 4 * s + lq) * ldl];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(p[s], x[kb][s], acc, 0, 0, 0);
+    }
+    v4d_t y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      y = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[i][ln][4 * s + lq], acc[s], y, 0, 0, 0);
+    x[i] = y;
+  }
+  if (cv) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bcol[16 * i + 4 * r + lq] = x[i][r];
+  }
+}
+
 }  // namespace
 
 int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long ldl, double* A12,
@@ -105,8 +177,8 @@ int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long l
   if (ncols <= 0 || nb <= 0) return FLA_HIP_OK;
   if (nb > TB) return FLA_HIP_EARG;
   if (nb == TB)
-    hipLaunchKernelGGL(trsm_llnu_kernel<true>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols, nb,
-                       L11, ldl, A12, lda);
+    hipLaunchKernelGGL(trsm_llnu128_kernel, dim3((ncols + 63) / 64), dim3(256), 0, s, ncols, L11,
+                       ldl, A12, lda);
   else
     hipLaunchKernelGGL(trsm_llnu_kernel<false>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols,
                        nb, L11, ldl, A12, lda);
