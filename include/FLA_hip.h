@@ -86,7 +86,10 @@ int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, const int* p, 
 int fla_hip_shift_pivots(int ptype, int m, int* p);
 
 /* ---- tuning ---- */
-/* Outer (trailing-update) and inner (panel) block sizes; 0 keeps current. */
+/* Block sizes; 0 keeps current.  chol_nb: outer Cholesky block (multiple of
+   128, <= 1024).  lu_nb: 32..128 (multiple of 32) = one panel of that width
+   per step; 256 (default) = blocks of two 128-column panels, whose trailing
+   update contracts over 256. */
 int fla_hip_set_blocksizes(int chol_nb, int lu_nb);
 int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb);
 /* Outer Cholesky block used for order n (chol_nb 0 = automatic: n/32

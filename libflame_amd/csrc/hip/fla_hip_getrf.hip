@@ -592,9 +592,9 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 }  // namespace
 
 // Exchange area of the panel kernel (one, shared by every launch: the tags
-// keep launches apart) and two move lists, alternated between steps, since
-// the previous step's row gathers may still read theirs while the next panel
-// runs.
+// keep launches apart) and four move lists: a panel must not reset a list
+// that row gathers still queued on the update stream read (two per 256-column
+// block, alternated between blocks; the 128-column path alternates two).
 struct LuBufs {
   PanelComm cm;
   int* moved;
@@ -606,10 +606,10 @@ constexpr size_t kHdrBytes = sizeof(u64x2) * 2 * MAX_P * HW;
 constexpr size_t kRowsBytes = sizeof(u64x2) * 2 * MAX_P * 3 * PW;
 constexpr size_t kJrowBytes = sizeof(u64x2) * 2 * PW;
 constexpr size_t kMovedBytes = sizeof(int) * (1 + 4 * PW);
-static_assert(kHdrBytes + kRowsBytes + kJrowBytes + 256 * 8 + 2 * kMovedBytes < kUcLuBytes,
+static_assert(kHdrBytes + kRowsBytes + kJrowBytes + 256 * 8 + 4 * kMovedBytes < kUcLuBytes,
               "LU exchange area");
 
-static void lu_carve(char* p, LuBufs (&b)[2]) {
+static void lu_carve(char* p, LuBufs (&b)[4]) {
   PanelComm cm;
   cm.hdr = reinterpret_cast<u64x2*>(p);
   p += align256(kHdrBytes);
@@ -618,7 +618,7 @@ static void lu_carve(char* p, LuBufs (&b)[2]) {
   cm.jrow = reinterpret_cast<u64x2*>(p);
   p += align256(kJrowBytes);
   cm.err = nullptr;  // set by the caller
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < 4; ++s) {
     b[s].cm = cm;
     b[s].moved = reinterpret_cast<int*>(p);
     p += align256(kMovedBytes);
@@ -690,36 +690,13 @@ static int lu_update_cols(hipStream_t s, int m, double* A, long lda, int k, int 
   return FLA_HIP_OK;
 }
 
-extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* info) {
-  FLA_TRY(fla_ensure_ready());
-  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
-  *info = kSuccess;
+// Blocks of one panel of width nb <= 128 (look-ahead over panels): the
+// next panel's columns first, then the remaining columns.
+static int lu_blocked1(FlaHipCtx& c, int m, int n, double* A, long lda, int* p, int* dinfo,
+                       LuBufs (&bufs)[4], double* Ut, long ldu) {
   const int mn = m < n ? m : n;
-  if (mn == 0) return FLA_HIP_OK;
-  FlaHipCtx& c = fla_ctx();
   const int nb = c.lu_nb <= PW ? c.lu_nb : PW;
-  const int Pmax = (m + ROWS_PER_WG - 1) / ROWS_PER_WG;
-  // Every panel workgroup must be resident at once (one per CU at most).
-  if (Pmax > c.num_cus || Pmax > MAX_P) {
-    std::fprintf(stderr, "libflame_amd: dgetrf supports m <= %d on this device\n",
-                 c.num_cus * ROWS_PER_WG);
-    return FLA_HIP_EARG;
-  }
-  // exchange area in the uncached region, laid out for MAX_P workgroups
-  LuBufs bufs[2];
-  lu_carve(c.uc, bufs);
-  // transposed U12 of one step (n x nb, ld n)
-  void* wut = nullptr;
-  FLA_TRY(fla_workspace((size_t)(n > 1 ? n : 1) * PW * sizeof(double), &wut));
-  double* Ut = static_cast<double*>(wut);
-  const long ldu = n > 1 ? n : 1;
-  int* dinfo = c.dscratch;
-  int* derr = c.dscratch + 4;  // one wait-failure flag for the whole factorization
-  bufs[0].cm.err = derr;
-  bufs[1].cm.err = derr;
   hipStream_t su = c.s_update, sq = c.s_panel;
-  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, su, dinfo, kSuccess);
-  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, su, derr, 0);
   int evi = 0;
   auto next_event = [&]() {
     hipEvent_t e = c.ev[evi];
@@ -758,6 +735,149 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
   hipEvent_t fin = next_event();
   FLA_HIP_CHECK(hipEventRecord(fin, sq));
   FLA_HIP_CHECK(hipStreamWaitEvent(su, fin, 0));
+  return FLA_HIP_OK;
+}
+
+// The same for a 256-column block factored as two panels, a at (k, k) of
+// width kba and b at (k + kba, k + kba) of width kbb: both panels'
+// interchanges, U12 := inv(L11) A12 with L11 = [La 0; Lba Lb] (two 128
+// solves around Bb -= Lba Xa), and one update with K = kba + kbb.
+static int lu_update_cols2(hipStream_t s, int m, double* A, long lda, int k, int kba, int kbb,
+                           int c0, int nc, const int* moved_a, const int* moved_b, double* Ut,
+                           long ldu) {
+  if (nc <= 0) return FLA_HIP_OK;
+  if (kbb <= 0) return lu_update_cols(s, m, A, lda, k, kba, c0, nc, moved_a, Ut, ldu);
+  const int kb = kba + kbb, kb1 = k + kba;
+  FLA_TRY(lu_gather(s, nc, A + (long)k + (long)c0 * lda, lda, moved_a));
+  FLA_TRY(lu_gather(s, nc, A + (long)kb1 + (long)c0 * lda, lda, moved_b));
+  double* Xa = A + (long)k + (long)c0 * lda;
+  double* Xb = A + (long)kb1 + (long)c0 * lda;
+  FLA_TRY(launch_trsm_llnu(s, nc, kba, A + (long)k + (long)k * lda, lda, Xa, lda));
+  FLA_TRY(launch_dgemm_mfma(s, false, false, 0, kbb, nc, kba, -1.0, A + (long)kb1 + (long)k * lda,
+                            lda, Xa, lda, 1.0, Xb, lda, nullptr));
+  FLA_TRY(launch_trsm_llnu(s, nc, kbb, A + (long)kb1 + (long)kb1 * lda, lda, Xb, lda));
+  const int mr = m - k - kb;
+  if (mr > 0) {
+    hipLaunchKernelGGL(transpose_kernel, dim3((kb + 31) / 32, (nc + 31) / 32), dim3(256), 0, s, kb,
+                       nc, Xa, lda, Ut, ldu);
+    FLA_HIP_CHECK(hipGetLastError());
+    void* tok = fla_stats_begin(s);
+    FLA_TRY(launch_dgemm_mfma(s, false, true, 0, mr, nc, kb, -1.0, A + (long)(k + kb) + (long)k * lda,
+                              lda, Ut, ldu, 1.0, A + (long)(k + kb) + (long)c0 * lda, lda, nullptr));
+    fla_stats_end(s, tok, 1, 2.0 * mr * (double)nc * kb);
+  }
+  return FLA_HIP_OK;
+}
+
+// 256-column blocks ("pairs") of two 128-column panels A and B.  The update
+// stream carries the chain: panel A; B's columns updated with A; panel B; B's
+// interchanges on A's columns; the next pair's 256 columns updated with the
+// pair (K = 256), after which the next panel A starts.  The bulk stream
+// updates every column beyond the next pair (K = 256) and applies the pair's
+// interchanges left of it, concurrently with the next pair's two panels; the
+// chain waits for it only before the following pair's columns.  Ut holds the
+// transposed U12 of the bulk update in rows [0, n) and of the chain's updates
+// in rows [n, n + 256) (ld ldu >= n + 256).
+static int lu_blocked2(FlaHipCtx& c, int m, int n, double* A, long lda, int* p, int* dinfo,
+                       LuBufs (&bufs)[4], double* Ut, long ldu) {
+  const int mn = m < n ? m : n;
+  hipStream_t su = c.s_update, sb = c.s_bulk, sq = c.s_panel;
+  double* Utn = Ut + n;
+  int evi = 0;
+  auto next_event = [&]() {
+    hipEvent_t e = c.ev[evi];
+    evi = (evi + 1) % FlaHipCtx::kEvents;
+    return e;
+  };
+  hipEvent_t start = next_event();
+  FLA_HIP_CHECK(hipEventRecord(start, su));
+  FLA_HIP_CHECK(hipStreamWaitEvent(sq, start, 0));
+  FLA_HIP_CHECK(hipStreamWaitEvent(sb, start, 0));
+  int blk = 0;
+  int kba = mn < PW ? mn : PW;
+  hipEvent_t bulk_done = nullptr;
+  FLA_TRY(lu_panel(sq, m, kba, A, lda, 0, p, dinfo, bufs[0]));
+  for (int k = 0;; ++blk) {
+    const LuBufs& ba = bufs[2 * (blk & 1)];
+    const LuBufs& bb = bufs[2 * (blk & 1) + 1];
+    hipEvent_t pa = next_event();
+    FLA_HIP_CHECK(hipEventRecord(pa, sq));
+    FLA_HIP_CHECK(hipStreamWaitEvent(su, pa, 0));
+    const int kb1 = k + kba;
+    const int kbb = (mn - kb1) < PW ? (mn - kb1) : PW;
+    if (kbb > 0) {
+      FLA_TRY(lu_update_cols(su, m, A, lda, k, kba, kb1, kbb, ba.moved, Utn, ldu));
+      hipEvent_t rb = next_event();
+      FLA_HIP_CHECK(hipEventRecord(rb, su));
+      FLA_HIP_CHECK(hipStreamWaitEvent(sq, rb, 0));
+      FLA_TRY(lu_panel(sq, m - kb1, kbb, A + (long)kb1 + (long)kb1 * lda, lda, kb1, p, dinfo, bb));
+      hipEvent_t pb = next_event();
+      FLA_HIP_CHECK(hipEventRecord(pb, sq));
+      FLA_HIP_CHECK(hipStreamWaitEvent(su, pb, 0));
+      FLA_TRY(lu_gather(su, kba, A + (long)kb1 + (long)k * lda, lda, bb.moved));
+    }
+    const int k2 = kb1 + (kbb > 0 ? kbb : 0);  // the next pair
+    const int nw = (mn - k2) < 2 * PW ? (mn - k2) : 2 * PW;
+    if (bulk_done) FLA_HIP_CHECK(hipStreamWaitEvent(su, bulk_done, 0));
+    FLA_TRY(lu_update_cols2(su, m, A, lda, k, kba, kbb, k2, nw > 0 ? nw : 0, ba.moved, bb.moved,
+                            Utn, ldu));
+    hipEvent_t ready = next_event();
+    FLA_HIP_CHECK(hipEventRecord(ready, su));
+    const int c0 = k2 + (nw > 0 ? nw : 0);
+    FLA_HIP_CHECK(hipStreamWaitEvent(sb, ready, 0));
+    FLA_TRY(lu_update_cols2(sb, m, A, lda, k, kba, kbb, c0, n - c0, ba.moved, bb.moved, Ut, ldu));
+    FLA_TRY(lu_gather(sb, k, A + (long)k, lda, ba.moved));
+    if (kbb > 0) FLA_TRY(lu_gather(sb, k, A + (long)kb1, lda, bb.moved));
+    bulk_done = next_event();
+    FLA_HIP_CHECK(hipEventRecord(bulk_done, sb));
+    if (nw <= 0) break;
+    const int kbn = nw < PW ? nw : PW;
+    FLA_HIP_CHECK(hipStreamWaitEvent(sq, ready, 0));
+    FLA_TRY(lu_panel(sq, m - k2, kbn, A + (long)k2 + (long)k2 * lda, lda, k2, p, dinfo,
+                     bufs[2 * ((blk + 1) & 1)]));
+    k = k2;
+    kba = kbn;
+  }
+  hipEvent_t fin = next_event();
+  FLA_HIP_CHECK(hipEventRecord(fin, sq));
+  FLA_HIP_CHECK(hipStreamWaitEvent(su, fin, 0));
+  FLA_HIP_CHECK(hipStreamWaitEvent(su, bulk_done, 0));
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* info) {
+  FLA_TRY(fla_ensure_ready());
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
+  *info = kSuccess;
+  const int mn = m < n ? m : n;
+  if (mn == 0) return FLA_HIP_OK;
+  FlaHipCtx& c = fla_ctx();
+  const int Pmax = (m + ROWS_PER_WG - 1) / ROWS_PER_WG;
+  // Every panel workgroup must be resident at once (one per CU at most).
+  if (Pmax > c.num_cus || Pmax > MAX_P) {
+    std::fprintf(stderr, "libflame_amd: dgetrf supports m <= %d on this device\n",
+                 c.num_cus * ROWS_PER_WG);
+    return FLA_HIP_EARG;
+  }
+  // exchange area in the uncached region, laid out for MAX_P workgroups
+  LuBufs bufs[4];
+  lu_carve(c.uc, bufs);
+  // transposed U12 of a step: (n + 2 PW) x 2 PW, ld n + 2 PW
+  void* wut = nullptr;
+  const long ldu = (long)n + 2 * PW;
+  FLA_TRY(fla_workspace((size_t)ldu * 2 * PW * sizeof(double), &wut));
+  double* Ut = static_cast<double*>(wut);
+  int* dinfo = c.dscratch;
+  int* derr = c.dscratch + 4;  // one wait-failure flag for the whole factorization
+  for (auto& b : bufs) b.cm.err = derr;
+  hipStream_t su = c.s_update, sq = c.s_panel;
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, su, dinfo, kSuccess);
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, su, derr, 0);
+  if (c.lu_nb > PW) {
+    FLA_TRY(lu_blocked2(c, m, n, A, lda, p, dinfo, bufs, Ut, ldu));
+  } else {
+    FLA_TRY(lu_blocked1(c, m, n, A, lda, p, dinfo, bufs, Ut, ldu));
+  }
   int h[2] = {kSuccess, 0};
   FLA_HIP_CHECK(hipMemcpyAsync(&h[0], dinfo, sizeof(int), hipMemcpyDeviceToHost, su));
   FLA_HIP_CHECK(hipMemcpyAsync(&h[1], derr, sizeof(int), hipMemcpyDeviceToHost, su));
@@ -770,7 +890,7 @@ extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* inf
 
 // ---- building blocks of the reduced-precision LU (fla_hip_mixed.hip) ----------
 
-static LuBufs g_mp_bufs[2];
+static LuBufs g_mp_bufs[4];
 
 int fla_lu_panel_setup(int m, int* derr) {
   FlaHipCtx& c = fla_ctx();
@@ -781,8 +901,7 @@ int fla_lu_panel_setup(int m, int* derr) {
     return FLA_HIP_EARG;
   }
   lu_carve(c.uc, g_mp_bufs);
-  g_mp_bufs[0].cm.err = derr;
-  g_mp_bufs[1].cm.err = derr;
+  for (auto& b : g_mp_bufs) b.cm.err = derr;
   return FLA_HIP_OK;
 }
 

@@ -35,6 +35,7 @@ struct FlaHipCtx {
   // Trailing updates of the reduced-precision factor: their panel chain, not
   // the update, is the critical path, so this stream leaves more CUs free.
   hipStream_t s_update_lp = nullptr;
+  hipStream_t s_bulk = nullptr;  // LU: bulk trailing updates beside the panel pair's chain
   static constexpr int kEvents = 8;
   hipEvent_t ev[kEvents] = {};
   // Grow-only device workspaces: slot 0 general scratch (dtrsm, dlaswp),
@@ -56,7 +57,7 @@ struct FlaHipCtx {
   double* dlinv = nullptr;
   int chol_nb = 0;  // 0: chosen from n (fla_chol_nb)
   int mp_nb = 0;    // reduced-precision factor block; 0: fla_chol_nb
-  int lu_nb = 128;
+  int lu_nb = 256;  // <= 128: panel width; 256: blocks of two 128-column panels
 };
 
 constexpr size_t kUcLuBytes = 4u << 20;

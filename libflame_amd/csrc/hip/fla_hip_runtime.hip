@@ -3,8 +3,9 @@
 //
 // The reference binds one rocBLAS handle per SuperMatrix thread and device
 // (FLASH_Queue_bind_hip, src/base/flamec/supermatrix/hip/main/FLASH_Queue_hip.c:297-318);
-// here one process drives one GPU through two streams: s_update for trailing
-// updates and s_panel for the look-ahead panel chain.
+// here one process drives one GPU through a few streams: s_update for trailing
+// updates, s_panel for the look-ahead panel chain, s_bulk for the LU's bulk
+// updates beside them, s_update_lp for the reduced-precision factor.
 #include <cstring>
 
 #include "fla_hip_internal.h"
@@ -23,6 +24,11 @@ static int g_reserved_cus = 8;
 // The reduced-precision factor's update stream leaves 64 CUs to its panels
 // (measured on MI355X at n = 16384).
 static int g_lp_reserved_cus = 64;
+// The LU's bulk-update stream leaves 64 CUs to the panel kernel, whose up to
+// 64 workgroups each need a whole CU and must all be resident (measured at
+// n = 16384: with 8 left, panels waited for CUs and took 119 ms instead of
+// 87 ms per factorization).
+static int g_bulk_reserved_cus = 64;
 
 static int create_masked_stream(hipStream_t* s, int reserved) {
   if (reserved <= 0 || reserved >= g_ctx.num_cus) {
@@ -72,6 +78,7 @@ extern "C" int fla_hip_init(void) {
   // Default-priority streams only.
   FLA_TRY(create_update_stream(g_reserved_cus));
   FLA_TRY(create_masked_stream(&g_ctx.s_update_lp, g_lp_reserved_cus));
+  FLA_TRY(create_masked_stream(&g_ctx.s_bulk, g_bulk_reserved_cus));
   FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_panel, hipStreamNonBlocking));
   for (int i = 0; i < FlaHipCtx::kEvents; ++i)
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
@@ -104,6 +111,7 @@ extern "C" int fla_hip_finalize(void) {
   for (int i = 0; i < FlaHipCtx::kEvents; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev[i]));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update_lp));
+  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_bulk));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_panel));
   g_ctx.ready = false;
   return FLA_HIP_OK;
@@ -113,6 +121,7 @@ extern "C" int fla_hip_synchronize(void) {
   FLA_TRY(fla_ensure_ready());
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_panel));
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update_lp));
+  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_bulk));
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
   return FLA_HIP_OK;
 }
@@ -235,7 +244,7 @@ extern "C" int fla_hip_set_blocksizes(int chol_nb, int lu_nb) {
     g_ctx.chol_nb = chol_nb;
   }
   if (lu_nb > 0) {
-    if (lu_nb % 32 != 0 || lu_nb > 128) return FLA_HIP_EARG;
+    if (lu_nb % 32 != 0 || (lu_nb > 128 && lu_nb != 256)) return FLA_HIP_EARG;
     g_ctx.lu_nb = lu_nb;
   }
   return FLA_HIP_OK;

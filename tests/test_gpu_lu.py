@@ -14,10 +14,14 @@ pytestmark = pytest.mark.gpu
 RESID_MAX = 10.0
 
 
-@pytest.fixture(scope="module", autouse=True)
-def _init():
+@pytest.fixture(scope="module", autouse=True, params=[256, 128, 64])
+def _init(request):
+    """Every test runs with 256-column blocks of two panels (the default),
+    with one 128-column panel per step, and with 64-column panels."""
     fla.FLA_Init()
+    fla.set_blocksizes(lu_nb=request.param)
     yield
+    fla.set_blocksizes(lu_nb=256)
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (5, 5), (16, 16), (127, 127), (128, 128), (129, 129),
