@@ -140,6 +140,7 @@ typedef struct FLA_Obj_view {
 #define FLA_OBJECT_NOT_FLOATING_POINT (-45)
 #define FLA_OBJECT_NOT_NONCONSTANT (-80)
 #define FLA_INVALID_VECTOR_DIM (-97)
+#define FLA_CHOL_FAILED_MATRIX_NOT_SPD (-67)
 #define FLA_ERROR_CODE_MIN (-10)
 #define FLA_ERROR_CODE_MAX (-111)
 /* libflame_amd codes, outside libflame's [-111, -10] range */
@@ -224,6 +225,9 @@ FLA_Error FLA_Chol_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj Lbar, FLA_Obj Abar)
 FLA_Error FLA_Chol_solve_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj X, FLA_Obj Xbar, FLA_Obj Bbar,
                                  FLA_Obj Abar);
 FLA_Error FLA_Shift_pivots_to(FLA_Pivot_type ptype, FLA_Obj p);
+FLA_Error FLA_Trinv(FLA_Uplo uplo, FLA_Diag diag, FLA_Obj A);
+FLA_Error FLA_Ttmm(FLA_Uplo uplo, FLA_Obj A);
+FLA_Error FLA_SPDinv(FLA_Uplo uplo, FLA_Obj A);
 FLA_Error FLA_Trsm_external(FLA_Side side, FLA_Uplo uplo, FLA_Trans trans, FLA_Diag diag,
                             FLA_Obj alpha, FLA_Obj A, FLA_Obj B);
 FLA_Error FLA_Herk_external(FLA_Uplo uplo, FLA_Trans trans, FLA_Obj alpha, FLA_Obj A,
@@ -248,6 +252,9 @@ int dsposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* 
             int* ldx, double* work, float* swork, int* iter, int* info);
 int dsgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb, double* x,
             int* ldx, double* work, float* swork, int* iter, int* info);
+int dtrtri_(char* uplo, char* diag, int* n, double* a, int* lda, int* info);
+int dlauum_(char* uplo, int* n, double* a, int* lda, int* info);
+int dpotri_(char* uplo, int* n, double* a, int* lda, int* info);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,18 @@ int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double 
 /* Blocked Cholesky, in place on the uplo triangle.  *info = FLA_SUCCESS (-1)
    or the 0-based index of the first column whose pivot is <= 0 or NaN. */
 int fla_hip_dpotrf(int uplo, int n, double* A, int lda, int* info);
+/* Triangular inverse in place on the uplo triangle, diag FLA_UNIT_DIAG or
+   FLA_NONUNIT_DIAG (the other triangle, and a unit diagonal, are not
+   touched).  Replaces FLA_Trinv (src/lapack/inv/tri/front/flamec/FLA_Trinv.c:16). */
+int fla_hip_dtrtri(int uplo, int diag, int n, double* A, int lda);
+/* A := L' L (lower) or U U' (upper) on the uplo triangle.  Replaces FLA_Ttmm
+   (src/lapack/misc/ttmm/front/flamec/FLA_Ttmm.c). */
+int fla_hip_dlauum(int uplo, int n, double* A, int lda);
+/* A := inv(A) for SPD A on the uplo triangle: Cholesky, triangular inverse,
+   then the triangle times its transpose.  *info = FLA_SUCCESS or the
+   0-based failing column of the Cholesky (A then holds the partial factor).
+   Replaces FLA_SPDinv (src/lapack/inv/spd/front/flamec/FLA_SPDinv_internal.c:22). */
+int fla_hip_dspdinv(int uplo, int n, double* A, int lda, int* info);
 /* Blocked LU with partial pivoting.  p (device, int[min(m,n)]) receives
    FLAME-native pivots (0-based, relative to the row).  *info = FLA_SUCCESS or
    the 0-based index of the first null pivot. */

@@ -79,6 +79,12 @@ def lib():
             "FLA_Chol_solve": (C.c_int, [C.c_int, O, O, O]),
             "FLA_LU_piv_solve": (C.c_int, [O, O, O, O]),
             "FLA_Shift_pivots_to": (C.c_int, [C.c_int, O]),
+            "FLA_Trinv": (C.c_int, [C.c_int, C.c_int, O]),
+            "FLA_Ttmm": (C.c_int, [C.c_int, O]),
+            "FLA_SPDinv": (C.c_int, [C.c_int, O]),
+            "dtrtri_": (C.c_int, [C.c_char_p, C.c_char_p, ip, dp, ip, ip]),
+            "dlauum_": (C.c_int, [C.c_char_p, ip, dp, ip, ip]),
+            "dpotri_": (C.c_int, [C.c_char_p, ip, dp, ip, ip]),
             "dpotrf_": (C.c_int, [C.c_char_p, ip, dp, ip, ip]),
             "dgetrf_": (C.c_int, [ip, ip, dp, ip, ip, ip]),
             "dpotrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, ip]),
@@ -114,6 +120,9 @@ def lib():
             "fla_hip_dtrsm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_double, vp, C.c_int, vp, C.c_int]),
             "fla_hip_dpotrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
+            "fla_hip_dtrtri": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_dlauum": (C.c_int, [C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_dspdinv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
             "fla_hip_dgetrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, ip]),
             "fla_hip_dlaswp": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int]),
             "fla_hip_shift_pivots": (C.c_int, [C.c_int, C.c_int, vp]),
@@ -310,6 +319,18 @@ def FLA_Finalize():
 
 def FLA_Chol(uplo, A):
     return lib().FLA_Chol(uplo, A.o if isinstance(A, Obj) else A)
+
+
+def FLA_Trinv(uplo, diag, A):
+    return lib().FLA_Trinv(uplo, diag, A.o if isinstance(A, Obj) else A)
+
+
+def FLA_Ttmm(uplo, A):
+    return lib().FLA_Ttmm(uplo, A.o if isinstance(A, Obj) else A)
+
+
+def FLA_SPDinv(uplo, A):
+    return lib().FLA_SPDinv(uplo, A.o if isinstance(A, Obj) else A)
 
 
 def FLA_LU_piv(A, p):

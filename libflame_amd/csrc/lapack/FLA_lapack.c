@@ -142,6 +142,77 @@ FLA_Error FLA_Chol(FLA_Uplo uplo, FLA_Obj A) {
   return info;
 }
 
+/* ---- FLA_Trinv / FLA_Ttmm / FLA_SPDinv ------------------------------------
+ * FLA_Trinv: src/lapack/inv/tri/front/flamec/FLA_Trinv.c:16-46 (checks
+ * FLA_Trinv_check.c:13-33; FLA_NONUNIT_DIAG selects the non-unit variants,
+ * anything else the unit ones, FLA_Trinv_internal.c; always FLA_SUCCESS).
+ * FLA_Ttmm: src/lapack/misc/ttmm/front/flamec/FLA_Ttmm.c (always FLA_SUCCESS).
+ * FLA_SPDinv: FLA_SPDinv_internal.c:22-39 -- Chol, then (a failed Cholesky is
+ * FLA_CHOL_FAILED_MATRIX_NOT_SPD through FLA_Check_error_code at error level
+ * >= MIN) Trinv (non-unit) and Ttmm; returns FLA_SUCCESS. */
+static FLA_Error square_checks(FLA_Uplo uplo, FLA_Obj A) {
+  if (CHECK_LEVEL_MIN) {
+    FLA_Check_error_code(fla_check_valid_uplo(uplo));
+    FLA_Check_error_code(fla_check_floating_object(A));
+    FLA_Check_error_code(fla_check_nonconstant_object(A));
+    FLA_Check_error_code(fla_check_square(A));
+  }
+  return FLA_SUCCESS;
+}
+
+FLA_Error FLA_Trinv(FLA_Uplo uplo, FLA_Diag diag, FLA_Obj A) {
+  fla_devmat dA;
+  FLA_Error e;
+  int st;
+  if (CHECK_LEVEL_MIN) FLA_Check_error_code(fla_check_valid_diag(diag));
+  square_checks(uplo, A);
+  if (FLA_Obj_has_zero_dim(A)) return FLA_SUCCESS;
+  if ((e = require_double(A)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
+  st = fla_hip_dtrtri(uplo, diag == FLA_NONUNIT_DIAG ? FLA_NONUNIT_DIAG : FLA_UNIT_DIAG, dA.m,
+                      (double*)dA.dev, dA.ld);
+  e = fla_devmat_release(&dA, st == FLA_HIP_OK);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_Trinv");
+  if (e != FLA_SUCCESS) return e;
+  return FLA_SUCCESS;
+}
+
+FLA_Error FLA_Ttmm(FLA_Uplo uplo, FLA_Obj A) {
+  fla_devmat dA;
+  FLA_Error e;
+  int st;
+  square_checks(uplo, A);
+  if (FLA_Obj_has_zero_dim(A)) return FLA_SUCCESS;
+  if ((e = require_double(A)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
+  st = fla_hip_dlauum(uplo, dA.m, (double*)dA.dev, dA.ld);
+  e = fla_devmat_release(&dA, st == FLA_HIP_OK);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_Ttmm");
+  if (e != FLA_SUCCESS) return e;
+  return FLA_SUCCESS;
+}
+
+FLA_Error FLA_SPDinv(FLA_Uplo uplo, FLA_Obj A) {
+  fla_devmat dA;
+  FLA_Error e;
+  int info = FLA_SUCCESS, st;
+  square_checks(uplo, A);
+  if (FLA_Obj_has_zero_dim(A)) return FLA_SUCCESS;
+  if ((e = require_double(A)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
+  st = fla_hip_dpotrf(uplo, dA.m, (double*)dA.dev, dA.ld, &info);
+  if (st == FLA_HIP_OK && info != FLA_SUCCESS && CHECK_LEVEL_MIN) {
+    fla_devmat_release(&dA, 1);
+    FLA_Check_error_code(FLA_CHOL_FAILED_MATRIX_NOT_SPD);
+  }
+  if (st == FLA_HIP_OK) st = fla_hip_dtrtri(uplo, FLA_NONUNIT_DIAG, dA.m, (double*)dA.dev, dA.ld);
+  if (st == FLA_HIP_OK) st = fla_hip_dlauum(uplo, dA.m, (double*)dA.dev, dA.ld);
+  e = fla_devmat_release(&dA, st == FLA_HIP_OK);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_SPDinv");
+  if (e != FLA_SUCCESS) return e;
+  return FLA_SUCCESS;
+}
+
 /* ---- FLA_LU_piv ---------------------------------------------------------- */
 FLA_Error FLA_LU_piv(FLA_Obj A, FLA_Obj p) {
   fla_devmat dA, dp;

@@ -8,6 +8,9 @@
  *                       FLA_Shift_pivots_to(FLA_LAPACK_PIVOTS))
  *   dpotrs_            src/map/common/lapacksrc/f2c/dpotrs.c:102-194
  *   dgetrs_            src/map/common/lapacksrc/f2c/dgetrs.c:115-210
+ *   dtrtri_ / dlauum_ / dpotri_
+ *                      src/map/lapack2flamec/FLA_trtri.c, FLA_lauum.c,
+ *                      FLA_potri.c (FLA_Trinv / FLA_Ttmm on the factor)
  *
  * The buffers are host memory (the LAPACK ABI); FLA_Chol / FLA_LU_piv stage
  * them through device memory.  Argument errors follow the netlib checks
@@ -322,4 +325,83 @@ int dsgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* l
     *info = -100;
   }
   return 0;
+}
+
+/* dtrtri_: src/map/lapack2flamec/FLA_trtri.c:33-71 after dtrtri_check
+ * (src/map/common/lapackcheck/dtrtri_check.c:3-61): argument checks, then,
+ * for a non-unit diagonal, the first exact zero on the diagonal is returned
+ * as info = i (1-based) and nothing is computed; else FLA_Trinv, info = 0. */
+int dtrtri_(char* uplo, char* diag, int* n, double* a, int* lda, int* info) {
+  FLA_Obj A;
+  FLA_Error init_result;
+  int u = up(*uplo), d = up(*diag), i;
+  *info = 0;
+  if (u != 'U' && u != 'L')
+    *info = -1;
+  else if (d != 'N' && d != 'U')
+    *info = -2;
+  else if (*n < 0)
+    *info = -3;
+  else if (*lda < imax(1, *n))
+    *info = -5;
+  if (*info != 0) {
+    xerbla("DTRTRI", -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  if (d == 'N')
+    for (i = 1; i <= *n; ++i)
+      if (a[(size_t)(i - 1) + (size_t)(i - 1) * (size_t)*lda] == 0.0) {
+        *info = i;
+        return 0;
+      }
+  FLA_Init_safe(&init_result);
+  FLA_Obj_create_without_buffer(FLA_DOUBLE, (dim_t)*n, (dim_t)*n, &A);
+  FLA_Obj_attach_buffer(a, 1, (dim_t)*lda, &A);
+  FLA_Trinv(u == 'L' ? FLA_LOWER_TRIANGULAR : FLA_UPPER_TRIANGULAR,
+            d == 'N' ? FLA_NONUNIT_DIAG : FLA_UNIT_DIAG, A);
+  FLA_Obj_free_without_buffer(&A);
+  FLA_Finalize_safe(init_result);
+  *info = 0;
+  return 0;
+}
+
+/* dlauum_ (FLA_lauum.c, dlauum_check.c: -1 uplo, -2 n, -4 lda) and dpotri_
+ * (FLA_potri.c:32-56, dpotri_check.c: the same checks): FLA_Ttmm, and
+ * FLA_Trinv (non-unit) then FLA_Ttmm, on the factor in place; info = 0. */
+static int inv_body(const char* name, int potri, char* uplo, int* n, double* a, int* lda,
+                    int* info) {
+  FLA_Obj A;
+  FLA_Error init_result;
+  int u = up(*uplo);
+  FLA_Uplo ufla;
+  *info = 0;
+  if (u != 'U' && u != 'L')
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*lda < imax(1, *n))
+    *info = -4;
+  if (*info != 0) {
+    xerbla(name, -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  ufla = u == 'L' ? FLA_LOWER_TRIANGULAR : FLA_UPPER_TRIANGULAR;
+  FLA_Init_safe(&init_result);
+  FLA_Obj_create_without_buffer(FLA_DOUBLE, (dim_t)*n, (dim_t)*n, &A);
+  FLA_Obj_attach_buffer(a, 1, (dim_t)*lda, &A);
+  if (potri) FLA_Trinv(ufla, FLA_NONUNIT_DIAG, A);
+  FLA_Ttmm(ufla, A);
+  FLA_Obj_free_without_buffer(&A);
+  FLA_Finalize_safe(init_result);
+  return 0;
+}
+
+int dlauum_(char* uplo, int* n, double* a, int* lda, int* info) {
+  return inv_body("DLAUUM", 0, uplo, n, a, lda, info);
+}
+
+int dpotri_(char* uplo, int* n, double* a, int* lda, int* info) {
+  return inv_body("DPOTRI", 1, uplo, n, a, lda, info);
 }

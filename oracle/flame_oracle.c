@@ -10,6 +10,8 @@
  *   FLA_LU_piv      fla_lu_piv_cntl2: blk var5 (b=128) -> blk var5 (b=16) -> opt var4
  *                   src/base/flamec/control/init/lapack/fla/FLA_LU_piv_cntl_init.c:26-88
  *   leaf BLAS       the builtin f2c reference BLAS, src/flablas/f2c/ (ddot.c, ...)
+ *   FLA_Trinv / FLA_Ttmm / FLA_SPDinv   the leaf sweeps (opt var3 / opt var2)
+ *                   over the whole matrix; see or_trinv / or_ttmm below
  *
  * Compile with -ffp-contract=off so that every a*b+c rounds twice, as the
  * reference's gcc build on x86-64 (no FMA in the baseline ISA) does.
@@ -507,6 +509,91 @@ static int chol_rec(int uplo, int n, double* a, int lda, const int* nb, int nlev
 int or_chol(int uplo, int n, double* a, int lda, const int* nb, int nlev)
 {
     return chol_rec(uplo, n, a, lda, nb, nlev);
+}
+
+/* ---- FLA_Trinv / FLA_Ttmm / FLA_SPDinv ---------------------------------- */
+
+/* Exchanges the strictly upper and strictly lower triangles: an upper-stored
+   triangle becomes the lower-stored transpose (and back). */
+static void tri_swap(int n, double* a, int lda)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = j + 1; i < n; ++i) {
+            const double t = A_(i, j);
+            A_(i, j) = A_(j, i);
+            A_(j, i) = t;
+        }
+}
+
+/* FLA_Trinv on the lower triangle, one unblocked sweep in the order of the
+   reference's leaf FLA_Trinv_ln_opd_var3
+   (src/lapack/inv/tri/ln/flamec/FLA_Trinv_ln_opt_var3.c:137-176) and, for a
+   unit diagonal, FLA_Trinv_lu_opd_var3 (.../lu/flamec/FLA_Trinv_lu_opt_var3.c:
+   124-150); per row i:  a21 := a21 / (-alpha11) (unit: -a21),
+   A20 += a21 a10t (dger, columns with a zero a10t entry skipped),
+   a10t := a10t / alpha11, alpha11 := 1 / alpha11.  The reference runs this
+   leaf on 128-blocks inside FLA_Trinv_ln_blk_var3; the sweep here is the same
+   inverse with another rounding order.  Upper storage: the transposed
+   problem, FLA_Trinv(U) = FLA_Trinv(U')'. */
+void or_trinv(int uplo, int diag, int n, double* a, int lda)
+{
+    if (uplo == OR_UPPER) tri_swap(n, a, lda);
+    for (int i = 0; i < n; ++i) {
+        const double al = A_(i, i);
+        if (diag != 500) {
+            const double s = 1.0 / (-al);
+            for (int r = i + 1; r < n; ++r) A_(r, i) *= s;
+        } else {
+            for (int r = i + 1; r < n; ++r) A_(r, i) = -A_(r, i);
+        }
+        for (int c = 0; c < i; ++c) {
+            const double t = A_(i, c);
+            if (t == 0.0) continue;
+            for (int r = i + 1; r < n; ++r) A_(r, c) += A_(r, i) * t;
+        }
+        if (diag != 500) {
+            const double s = 1.0 / al;
+            for (int c = 0; c < i; ++c) A_(i, c) *= s;
+            A_(i, i) = 1.0 / al;
+        }
+    }
+    if (uplo == OR_UPPER) tri_swap(n, a, lda);
+}
+
+/* FLA_Ttmm on the lower triangle (A := L' L), in the order of the reference's
+   leaf FLA_Ttmm_l_opd_var2 (src/lapack/misc/ttmm/l/flamec/
+   FLA_Ttmm_l_opt_var2.c:136-176); per row i:  a10t := alpha11 a10t,
+   a10t += A20' a21 (dgemv 'T': one dot per column), alpha11 := alpha11^2,
+   alpha11 += a21' a21.  Upper storage: A := U U' through the transposed
+   problem. */
+void or_ttmm(int uplo, int n, double* a, int lda)
+{
+    if (uplo == OR_UPPER) tri_swap(n, a, lda);
+    for (int i = 0; i < n; ++i) {
+        const double al = A_(i, i);
+        for (int c = 0; c < i; ++c) A_(i, c) *= al;
+        for (int c = 0; c < i; ++c) {
+            double t = 0.0;
+            for (int r = i + 1; r < n; ++r) t += A_(r, c) * A_(r, i);
+            A_(i, c) += t;
+        }
+        double d = 0.0;
+        for (int r = i + 1; r < n; ++r) d += A_(r, i) * A_(r, i);
+        A_(i, i) = al * al + d;
+    }
+    if (uplo == OR_UPPER) tri_swap(n, a, lda);
+}
+
+/* FLA_SPDinv_internal (src/lapack/inv/spd/front/flamec/FLA_SPDinv_internal.c:
+   22-39): Cholesky (the oracle's FLA_Chol), then FLA_Trinv (non-unit) and
+   FLA_Ttmm.  Returns the Cholesky's result; on failure nothing else runs. */
+int or_spdinv(int uplo, int n, double* a, int lda, const int* nb, int nlev)
+{
+    const int r = or_chol(uplo, n, a, lda, nb, nlev);
+    if (r != OR_SUCCESS) return r;
+    or_trinv(uplo, 501, n, a, lda);
+    or_ttmm(uplo, n, a, lda);
+    return OR_SUCCESS;
 }
 
 /* FLA_Apply_pivots_ln_opd_var1 (src/lapack/util/app/piv/ln/flamec/

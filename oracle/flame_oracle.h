@@ -78,6 +78,12 @@ void or_gen_spd_c2(uint64_t seed, int n, double* a, int lda);
 /* LU: U[-1,1] with row i scaled by 10^(-i/n) (well-separated pivots). */
 void or_gen_lu_c2(uint64_t seed, int m, int n, double* a, int lda);
 
+/* ---- next-row widening: inverses ---- */
+/* FLA_Trinv (diag 500 unit / 501 non-unit), FLA_Ttmm, FLA_SPDinv. */
+void or_trinv(int uplo, int diag, int n, double* a, int lda);
+void or_ttmm(int uplo, int n, double* a, int lda);
+int or_spdinv(int uplo, int n, double* a, int lda, const int* nb, int nlev);
+
 /* ---- error metrics ---- */
 /* ||A - L*L^T||_1 / (n * eps * ||A||_1) using only the uplo triangle of A0. */
 double or_chol_residual(int uplo, int n, const double* a0, int lda0,

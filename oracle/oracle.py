@@ -78,6 +78,10 @@ def lib():
         for f in ("or_gen_spd_c1", "or_gen_spd_c2"):
             getattr(L, f).argtypes = [C.c_uint64, C.c_int, _dp, C.c_int]
         L.or_gen_lu_c2.argtypes = [C.c_uint64, C.c_int, C.c_int, _dp, C.c_int]
+        L.or_trinv.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int]
+        L.or_ttmm.argtypes = [C.c_int, C.c_int, _dp, C.c_int]
+        L.or_spdinv.restype = C.c_int
+        L.or_spdinv.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int]
         L.or_chol_residual.restype = C.c_double
         L.or_chol_residual.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int]
         L.or_lu_residual.restype = C.c_double
@@ -125,6 +129,25 @@ def chol(A, uplo=LOWER, nb=NB_CHOL):
     A = fortran(A).copy(order="F")
     nbv = np.array(nb, dtype=np.int32)
     r = lib().or_chol(uplo, A.shape[0], dp(A), max(1, A.shape[0]), ip(nbv), len(nb))
+    return A, r
+
+
+def trinv(A, uplo=LOWER, diag=501):
+    A = fortran(A).copy(order="F")
+    lib().or_trinv(uplo, diag, A.shape[0], dp(A), max(1, A.shape[0]))
+    return A
+
+
+def ttmm(A, uplo=LOWER):
+    A = fortran(A).copy(order="F")
+    lib().or_ttmm(uplo, A.shape[0], dp(A), max(1, A.shape[0]))
+    return A
+
+
+def spdinv(A, uplo=LOWER, nb=NB_CHOL):
+    A = fortran(A).copy(order="F")
+    nbv = np.array(nb, dtype=np.int32)
+    r = lib().or_spdinv(uplo, A.shape[0], dp(A), max(1, A.shape[0]), ip(nbv), len(nb))
     return A, r
 
 

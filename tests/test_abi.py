@@ -104,3 +104,30 @@ print(L.FLA_Chol(999, o))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip() == "-1"
+
+
+def test_inverse_lapack_argument_errors(capfd):
+    """dtrtri_ / dlauum_ / dpotri_ argument checks (netlib order) return
+    before any device work."""
+    import ctypes as C
+
+    import numpy as np
+
+    import libflame_amd as fla
+
+    L = fla.lib()
+    a = np.zeros((4, 4), order="F")
+    ap = a.ctypes.data_as(C.POINTER(C.c_double))
+    n, bad, info = (np.array([v], np.int32) for v in (4, 2, 0))
+    ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    L.dtrtri_(C.c_char_p(b"X"), C.c_char_p(b"N"), ip(n), ap, ip(n), ip(info))
+    assert info[0] == -1
+    L.dtrtri_(C.c_char_p(b"L"), C.c_char_p(b"Q"), ip(n), ap, ip(n), ip(info))
+    assert info[0] == -2
+    L.dtrtri_(C.c_char_p(b"L"), C.c_char_p(b"N"), ip(n), ap, ip(bad), ip(info))
+    assert info[0] == -5
+    L.dlauum_(C.c_char_p(b"L"), ip(np.array([-1], np.int32)), ap, ip(n), ip(info))
+    assert info[0] == -2
+    L.dpotri_(C.c_char_p(b"U"), ip(n), ap, ip(bad), ip(info))
+    assert info[0] == -4
+    assert "DTRTRI" in capfd.readouterr().err
