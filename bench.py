@@ -383,6 +383,40 @@ def bench_adjoint(fla, np, n):
     return out
 
 
+def bench_spdinv(fla, n):
+    """FLA_SPDinv of an n x n SPD matrix (A = B B'/n + I, lower triangle):
+    Cholesky, triangular inverse, L'L.  Flops n^3 (n^3/3 per phase).  The
+    phases are timed again one call at a time on a fresh copy."""
+    import ctypes as C
+    lib = fla.lib()
+    lo = fla.FLA_LOWER_TRIANGULAR
+    info = C.c_int()
+    ts = []
+    for rep in range(3):
+        A = make_spd(fla, n, 77)
+        t0 = time.perf_counter()
+        fla.check(lib.fla_hip_dspdinv(lo, n, A.ptr, n, C.byref(info)), "dspdinv")
+        fla.synchronize()
+        ts.append(time.perf_counter() - t0)
+        A.buf.free()
+    t = min(ts[1:])
+    A = make_spd(fla, n, 77)
+    phases = {}
+    for name, call in (("chol", lambda: lib.fla_hip_dpotrf(lo, n, A.ptr, n, C.byref(info))),
+                       ("trinv", lambda: lib.fla_hip_dtrtri(lo, fla.FLA_NONUNIT_DIAG, n, A.ptr, n)),
+                       ("ttmm", lambda: lib.fla_hip_dlauum(lo, n, A.ptr, n))):
+        t0 = time.perf_counter()
+        fla.check(call(), name)
+        fla.synchronize()
+        dt = time.perf_counter() - t0
+        phases[name] = {"ms": round(dt * 1e3, 2), "tflops": round(n ** 3 / 3 / dt / 1e12, 3)}
+    A.buf.free()
+    out = {"n": n, "ms": round(t * 1e3, 2), "tflops": round(n ** 3 / t / 1e12, 3),
+           "info": info.value, "phases": phases}
+    log(f"spdinv: {out}")
+    return out
+
+
 def bench_adjoint_batched(fla, np, count, n):
     """Batched FP32 GP gradient: count problems of n points in [0,1]^2, l_i
     uniform on [0.05, 0.5], s = 1, noise 1e-3, y ~ N(0,1) (one launch each for
@@ -437,6 +471,8 @@ def main():
     ap.add_argument("--mp-n", type=int, default=16384)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--adj-n", type=int, default=8192)
+    ap.add_argument("--no-inv", action="store_true")
+    ap.add_argument("--inv-n", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -540,6 +576,8 @@ def main():
         extras["batched"] = bench_batched(fla, np, args.batch_count)
     if not args.no_adjoint:
         extras["adjoint"] = bench_adjoint(fla, np, args.adj_n)
+    if not args.no_inv:
+        extras["spdinv"] = bench_spdinv(fla, args.inv_n)
 
     cpu = None if args.no_cpu else cpu_baseline(fla, args.cpu_n)
     out = {
