@@ -417,6 +417,34 @@ def bench_spdinv(fla, n):
     return out
 
 
+def bench_lu_nopiv(fla, np, n):
+    """FLA_LU_nopiv of an n x n diagonally dominant matrix (U(-1,1) entries
+    plus n on the diagonal, made on the host, uploaded before each timed
+    call); flops 2 n^3 / 3."""
+    import ctypes as C
+    lib = fla.lib()
+    info = C.c_int()
+    rng = np.random.default_rng(78)
+    H = np.asfortranarray(rng.uniform(-1.0, 1.0, size=(n, n)))
+    H[np.diag_indices(n)] += n
+    A = fla.DeviceMatrix(n, n)
+    ts = []
+    for rep in range(3):
+        A.upload(H)
+        fla.synchronize()
+        t0 = time.perf_counter()
+        fla.check(lib.fla_hip_dgetrf_nopiv(n, n, A.ptr, n, C.byref(info)), "dgetrf_nopiv")
+        fla.synchronize()
+        ts.append(time.perf_counter() - t0)
+    A.buf.free()
+    del H
+    t = min(ts[1:])
+    out = {"n": n, "ms": round(t * 1e3, 2), "tflops": round(2 * n ** 3 / 3 / t / 1e12, 3),
+           "info": info.value}
+    log(f"lu_nopiv: {out}")
+    return out
+
+
 def bench_adjoint_batched(fla, np, count, n):
     """Batched FP32 GP gradient: count problems of n points in [0,1]^2, l_i
     uniform on [0.05, 0.5], s = 1, noise 1e-3, y ~ N(0,1) (one launch each for
@@ -578,6 +606,7 @@ def main():
         extras["adjoint"] = bench_adjoint(fla, np, args.adj_n)
     if not args.no_inv:
         extras["spdinv"] = bench_spdinv(fla, args.inv_n)
+        extras["lu_nopiv"] = bench_lu_nopiv(fla, np, args.inv_n)
 
     cpu = None if args.no_cpu else cpu_baseline(fla, args.cpu_n)
     out = {

@@ -225,6 +225,8 @@ FLA_Error FLA_Chol_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj Lbar, FLA_Obj Abar)
 FLA_Error FLA_Chol_solve_adjoint(FLA_Uplo uplo, FLA_Obj L, FLA_Obj X, FLA_Obj Xbar, FLA_Obj Bbar,
                                  FLA_Obj Abar);
 FLA_Error FLA_Shift_pivots_to(FLA_Pivot_type ptype, FLA_Obj p);
+FLA_Error FLA_LU_nopiv(FLA_Obj A);
+FLA_Error FLA_LU_nopiv_solve(FLA_Obj A, FLA_Obj B, FLA_Obj X);
 FLA_Error FLA_Trinv(FLA_Uplo uplo, FLA_Diag diag, FLA_Obj A);
 FLA_Error FLA_Ttmm(FLA_Uplo uplo, FLA_Obj A);
 FLA_Error FLA_SPDinv(FLA_Uplo uplo, FLA_Obj A);

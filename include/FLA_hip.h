@@ -75,6 +75,11 @@ int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double 
 /* Blocked Cholesky, in place on the uplo triangle.  *info = FLA_SUCCESS (-1)
    or the 0-based index of the first column whose pivot is <= 0 or NaN. */
 int fla_hip_dpotrf(int uplo, int n, double* A, int lda, int* info);
+/* LU without pivoting of an m x n matrix in place (unit L below, U on and
+   above the diagonal).  *info = FLA_SUCCESS or the 0-based index of the
+   first exact zero on U's diagonal.  Replaces FLA_LU_nopiv
+   (src/lapack/dec/lu/nopiv/front/flamec/FLA_LU_nopiv.c:16). */
+int fla_hip_dgetrf_nopiv(int m, int n, double* A, int lda, int* info);
 /* Triangular inverse in place on the uplo triangle, diag FLA_UNIT_DIAG or
    FLA_NONUNIT_DIAG (the other triangle, and a unit diagonal, are not
    touched).  Replaces FLA_Trinv (src/lapack/inv/tri/front/flamec/FLA_Trinv.c:16). */

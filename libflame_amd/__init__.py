@@ -79,6 +79,8 @@ def lib():
             "FLA_Chol_solve": (C.c_int, [C.c_int, O, O, O]),
             "FLA_LU_piv_solve": (C.c_int, [O, O, O, O]),
             "FLA_Shift_pivots_to": (C.c_int, [C.c_int, O]),
+            "FLA_LU_nopiv": (C.c_int, [O]),
+            "FLA_LU_nopiv_solve": (C.c_int, [O, O, O]),
             "FLA_Trinv": (C.c_int, [C.c_int, C.c_int, O]),
             "FLA_Ttmm": (C.c_int, [C.c_int, O]),
             "FLA_SPDinv": (C.c_int, [C.c_int, O]),
@@ -120,6 +122,7 @@ def lib():
             "fla_hip_dtrsm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_double, vp, C.c_int, vp, C.c_int]),
             "fla_hip_dpotrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
+            "fla_hip_dgetrf_nopiv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
             "fla_hip_dtrtri": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int]),
             "fla_hip_dlauum": (C.c_int, [C.c_int, C.c_int, vp, C.c_int]),
             "fla_hip_dspdinv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
@@ -319,6 +322,14 @@ def FLA_Finalize():
 
 def FLA_Chol(uplo, A):
     return lib().FLA_Chol(uplo, A.o if isinstance(A, Obj) else A)
+
+
+def FLA_LU_nopiv(A):
+    return lib().FLA_LU_nopiv(A.o if isinstance(A, Obj) else A)
+
+
+def FLA_LU_nopiv_solve(A, B, X):
+    return lib().FLA_LU_nopiv_solve(A.o, B.o, X.o)
 
 
 def FLA_Trinv(uplo, diag, A):

@@ -189,6 +189,13 @@ int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, in
                             int* done, int* err);
 // Inverts the 128x128 diagonal blocks of a triangle (lower or upper storage)
 // into W, block k at W + k*128*128 (ld 128, A's orientation, other triangle 0).
+// Substitution (no block inverses, the reference's dtrsm_ order): A12 :=
+// inv(trilu(L11)) A12 one lane per column; A21 := A21 inv(triu(U11)) one lane
+// per row (nb <= 128).
+int launch_trsm_llnu_subst(hipStream_t s, int ncols, int nb, const double* L11, long ldl,
+                           double* A12, long lda);
+int launch_trsm_runn_rows(hipStream_t s, int nrows, int nb, const double* U11, long ldu,
+                          double* A21, long lda);
 int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool upper, bool unit,
                         double* W);
 // FP64 factor of columns [k, k+kb) of a lower matrix (rows k..n-1): diagonal

@@ -100,6 +100,35 @@ __global__ __launch_bounds__(64) void trsm_llnu_kernel(int ncols, int nb,
     if (FULL || i < nb) col[i] = x[i];
 }
 
+// A21 := A21 * inv(triu(U11)) for U11 upper non-unit of order nb <= 128 (ld
+// ldu): lane = row, forward over columns in dtrsm_('R','U','N','N') order
+// (src/flablas/f2c/dtrsm.c: b(i,j) -= a(k,j) b(i,k) for k < j with a(k,j) != 0,
+// then b(i,j) *= 1 / a(j,j)); rows are read and written coalesced.
+template <bool FULL>
+__global__ __launch_bounds__(64) void trsm_runn_rows_kernel(int nrows, int nb,
+                                                            const double* __restrict__ U, long ldu,
+                                                            double* __restrict__ B, long ldb) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= nrows) return;
+  double x[TB];
+#pragma unroll
+  for (int c = 0; c < TB; ++c) x[c] = (FULL || c < nb) ? B[(long)r + (long)c * ldb] : 0.0;
+#pragma unroll
+  for (int c = 0; c < TB; ++c) {
+    if (FULL || c < nb) {
+#pragma unroll
+      for (int k = 0; k < c; ++k) {
+        const double u = U[(long)k + (long)c * ldu];
+        if (u != 0.0) x[c] -= u * x[k];
+      }
+      x[c] *= 1.0 / U[(long)c + (long)c * ldu];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < TB; ++c)
+    if (FULL || c < nb) B[(long)r + (long)c * ldb] = x[c];
+}
+
 // A12 := inv(L11) * A12 for L11 unit lower of order 128 (ld ldl), in place:
 // one wave per 16 columns, 16-row blocks,
 //   X_i = inv(L_ii) (B_i - sum_{k<i} L_ik X_k),
@@ -182,6 +211,34 @@ int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long l
   else
     hipLaunchKernelGGL(trsm_llnu_kernel<false>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols,
                        nb, L11, ldl, A12, lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+int launch_trsm_llnu_subst(hipStream_t s, int ncols, int nb, const double* L11, long ldl,
+                           double* A12, long lda) {
+  if (ncols <= 0 || nb <= 0) return FLA_HIP_OK;
+  if (nb > TB) return FLA_HIP_EARG;
+  if (nb == TB)
+    hipLaunchKernelGGL(trsm_llnu_kernel<true>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols, nb,
+                       L11, ldl, A12, lda);
+  else
+    hipLaunchKernelGGL(trsm_llnu_kernel<false>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols,
+                       nb, L11, ldl, A12, lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+int launch_trsm_runn_rows(hipStream_t s, int nrows, int nb, const double* U11, long ldu,
+                          double* A21, long lda) {
+  if (nrows <= 0 || nb <= 0) return FLA_HIP_OK;
+  if (nb > TB) return FLA_HIP_EARG;
+  if (nb == TB)
+    hipLaunchKernelGGL(trsm_runn_rows_kernel<true>, dim3((nrows + 63) / 64), dim3(64), 0, s,
+                       nrows, nb, U11, ldu, A21, lda);
+  else
+    hipLaunchKernelGGL(trsm_runn_rows_kernel<false>, dim3((nrows + 63) / 64), dim3(64), 0, s,
+                       nrows, nb, U11, ldu, A21, lda);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

@@ -142,6 +142,65 @@ FLA_Error FLA_Chol(FLA_Uplo uplo, FLA_Obj A) {
   return info;
 }
 
+/* ---- FLA_LU_nopiv / FLA_LU_nopiv_solve ------------------------------------
+ * src/lapack/dec/lu/nopiv/front/flamec/FLA_LU_nopiv.c:16-36 (checks
+ * FLA_LU_nopiv_check.c: floating, nonconstant; at error level >= MIN the
+ * result is FLA_LU_find_zero_on_diagonal: the first exact zero on U's
+ * diagonal, else FLA_SUCCESS) and FLA_LU_nopiv_solve.c:13-32 (copy B to X,
+ * unit-lower then non-unit-upper triangular solves). */
+FLA_Error FLA_LU_nopiv(FLA_Obj A) {
+  fla_devmat dA;
+  FLA_Error e;
+  int info = FLA_SUCCESS, st;
+  if (CHECK_LEVEL_MIN) {
+    FLA_Check_error_code(fla_check_floating_object(A));
+    FLA_Check_error_code(fla_check_nonconstant_object(A));
+  }
+  if (FLA_Obj_has_zero_dim(A)) return FLA_SUCCESS;
+  if ((e = require_double(A)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
+  st = fla_hip_dgetrf_nopiv(dA.m, dA.n, (double*)dA.dev, dA.ld, &info);
+  e = fla_devmat_release(&dA, st == FLA_HIP_OK);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_LU_nopiv");
+  if (e != FLA_SUCCESS) return e;
+  return CHECK_LEVEL_MIN ? info : FLA_SUCCESS;
+}
+
+FLA_Error FLA_LU_nopiv_solve(FLA_Obj A, FLA_Obj B, FLA_Obj X) {
+  fla_devmat dA, dX;
+  FLA_Error e;
+  int st;
+  if (CHECK_LEVEL_MIN) {
+    FLA_Check_error_code(fla_check_floating_object(A));
+    FLA_Check_error_code(fla_check_nonconstant_object(A));
+    FLA_Check_error_code(fla_check_identical_object_datatype(A, B));
+    FLA_Check_error_code(fla_check_identical_object_datatype(A, X));
+    FLA_Check_error_code(fla_check_square(A));
+    FLA_Check_error_code(fla_check_matrix_matrix_dims(FLA_NO_TRANSPOSE, FLA_NO_TRANSPOSE, A, X, B));
+  }
+  if (FLA_Obj_is_identical(B, X) == FALSE) {
+    e = FLA_Copy(B, X);
+    if (e != FLA_SUCCESS) return e;
+  }
+  if (FLA_Obj_has_zero_dim(X)) return FLA_SUCCESS;
+  if ((e = require_double(A)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
+  if ((e = fla_devmat_acquire(X, 1, &dX)) != FLA_SUCCESS) {
+    fla_devmat_release(&dA, 0);
+    return e;
+  }
+  st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_UNIT_DIAG, dX.m, dX.n,
+                     1.0, (const double*)dA.dev, dA.ld, (double*)dX.dev, dX.ld);
+  if (st == FLA_HIP_OK)
+    st = fla_hip_dtrsm(FLA_LEFT, FLA_UPPER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_NONUNIT_DIAG, dX.m,
+                       dX.n, 1.0, (const double*)dA.dev, dA.ld, (double*)dX.dev, dX.ld);
+  if (st == FLA_HIP_OK) st = fla_hip_synchronize();
+  fla_devmat_release(&dA, 0);
+  e = fla_devmat_release(&dX, st == FLA_HIP_OK);
+  if (st != FLA_HIP_OK) return gpu_status(st, "FLA_LU_nopiv_solve");
+  return e == FLA_SUCCESS ? FLA_SUCCESS : e;
+}
+
 /* ---- FLA_Trinv / FLA_Ttmm / FLA_SPDinv ------------------------------------
  * FLA_Trinv: src/lapack/inv/tri/front/flamec/FLA_Trinv.c:16-46 (checks
  * FLA_Trinv_check.c:13-33; FLA_NONUNIT_DIAG selects the non-unit variants,

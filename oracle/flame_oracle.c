@@ -511,6 +511,67 @@ int or_chol(int uplo, int n, double* a, int lda, const int* nb, int nlev)
     return chol_rec(uplo, n, a, lda, nb, nlev);
 }
 
+/* ---- FLA_LU_nopiv --------------------------------------------------------- */
+
+/* Leaf FLA_LU_nopiv_opd_var5 (src/lapack/dec/lu/nopiv/vars/flamec/
+   FLA_LU_nopiv_opt_var5.c:132-160): per column, a21 := a21 / alpha11
+   (bl1_dinvscalv: by the reciprocal, skipped for 1), then
+   A22 := A22 - a21 a12t (dger with alpha -1, columns with a zero a12t entry
+   skipped). */
+static void lu_nopiv_opd_var5(int m, int n, double* a, int lda)
+{
+    const int mn = imin(m, n);
+    for (int i = 0; i < mn; ++i) {
+        const double al = A_(i, i);
+        if (al != 1.0) {
+            const double s = 1.0 / al;
+            for (int r = i + 1; r < m; ++r) A_(r, i) *= s;
+        }
+        for (int c = i + 1; c < n; ++c) {
+            const double t = A_(i, c);
+            if (t == 0.0) continue;
+            const double tm = -1.0 * t;
+            for (int r = i + 1; r < m; ++r) A_(r, c) += A_(r, i) * tm;
+        }
+    }
+}
+
+/* FLA_LU_nopiv_blk_var5 (FLA_LU_nopiv_blk_var5.c:36-52), one level per
+   block size: A11 := LU(A11); A12 := trilu(A11) \ A12;
+   A21 := A21 / triu(A11); A22 -= A21 A12. */
+static void lu_nopiv_rec(int m, int n, double* a, int lda, const int* nb, int nlev)
+{
+    if (nlev <= 0) {
+        lu_nopiv_opd_var5(m, n, a, lda);
+        return;
+    }
+    const int mn = imin(m, n);
+    for (int k = 0; k < mn;) {
+        const int b = imin(mn - k, nb[0]);
+        const int nr = n - k - b, mr = m - k - b;
+        lu_nopiv_rec(b, b, &A_(k, k), lda, nb + 1, nlev - 1);
+        if (nr > 0) or_dtrsm('L', 'L', 'N', 'U', b, nr, 1.0, &A_(k, k), lda, &A_(k, k + b), lda);
+        if (mr > 0) or_dtrsm('R', 'U', 'N', 'N', mr, b, 1.0, &A_(k, k), lda, &A_(k + b, k), lda);
+        if (mr > 0 && nr > 0)
+            or_dgemm('N', 'N', mr, nr, b, -1.0, &A_(k + b, k), lda, &A_(k, k + b), lda, 1.0,
+                     &A_(k + b, k + b), lda);
+        k += b;
+    }
+}
+
+/* FLA_LU_nopiv (src/lapack/dec/lu/nopiv/front/flamec/FLA_LU_nopiv.c:16-36)
+   with fla_lu_nopiv_cntl2 (FLA_LU_nopiv_cntl_init.c:28-70: var5 at b, var5
+   at b/4, leaf opt var5), returning, as at error level >= MIN, the first
+   exact zero on the diagonal (FLA_LU_find_zero_on_diagonal.c:25-46) or
+   OR_SUCCESS. */
+int or_lu_nopiv(int m, int n, double* a, int lda, const int* nb, int nlev)
+{
+    lu_nopiv_rec(m, n, a, lda, nb, nlev);
+    for (int i = 0; i < imin(m, n); ++i)
+        if (A_(i, i) == 0.0) return i;
+    return OR_SUCCESS;
+}
+
 /* ---- FLA_Trinv / FLA_Ttmm / FLA_SPDinv ---------------------------------- */
 
 /* Exchanges the strictly upper and strictly lower triangles: an upper-stored

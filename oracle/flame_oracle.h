@@ -78,7 +78,9 @@ void or_gen_spd_c2(uint64_t seed, int n, double* a, int lda);
 /* LU: U[-1,1] with row i scaled by 10^(-i/n) (well-separated pivots). */
 void or_gen_lu_c2(uint64_t seed, int m, int n, double* a, int lda);
 
-/* ---- next-row widening: inverses ---- */
+/* ---- next-row widening: LU without pivoting, inverses ---- */
+/* FLA_LU_nopiv; returns the first exact zero on U's diagonal or OR_SUCCESS. */
+int or_lu_nopiv(int m, int n, double* a, int lda, const int* nb, int nlev);
 /* FLA_Trinv (diag 500 unit / 501 non-unit), FLA_Ttmm, FLA_SPDinv. */
 void or_trinv(int uplo, int diag, int n, double* a, int lda);
 void or_ttmm(int uplo, int n, double* a, int lda);

@@ -78,6 +78,8 @@ def lib():
         for f in ("or_gen_spd_c1", "or_gen_spd_c2"):
             getattr(L, f).argtypes = [C.c_uint64, C.c_int, _dp, C.c_int]
         L.or_gen_lu_c2.argtypes = [C.c_uint64, C.c_int, C.c_int, _dp, C.c_int]
+        L.or_lu_nopiv.restype = C.c_int
+        L.or_lu_nopiv.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int]
         L.or_trinv.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int]
         L.or_ttmm.argtypes = [C.c_int, C.c_int, _dp, C.c_int]
         L.or_spdinv.restype = C.c_int
@@ -129,6 +131,14 @@ def chol(A, uplo=LOWER, nb=NB_CHOL):
     A = fortran(A).copy(order="F")
     nbv = np.array(nb, dtype=np.int32)
     r = lib().or_chol(uplo, A.shape[0], dp(A), max(1, A.shape[0]), ip(nbv), len(nb))
+    return A, r
+
+
+def lu_nopiv(A, nb=NB_CHOL):
+    """FLA_LU_nopiv (blocks 128 and 32, as fla_lu_nopiv_cntl2)."""
+    A = fortran(A).copy(order="F")
+    nbv = np.array(nb, dtype=np.int32)
+    r = lib().or_lu_nopiv(A.shape[0], A.shape[1], dp(A), max(1, A.shape[0]), ip(nbv), len(nb))
     return A, r
 
 

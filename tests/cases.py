@@ -51,6 +51,37 @@ def exact_lu(n, seed, m=None, null_col=None):
     return np.asfortranarray(A), np.asfortranarray(L), np.asfortranarray(U), p
 
 
+def exact_lu_nopiv(n, seed, m=None, null_col=None):
+    """A = L U with the factors of exact_lu and no interchanges: LU without
+    pivoting recovers L and U (exactly under forward substitution; the GPU's
+    inverse-based A21 solve rounds).  With null_col, U[null_col, null_col] =
+    0: the first exact zero on the diagonal is there."""
+    m = n if m is None else m
+    k = min(m, n)
+    rng = np.random.default_rng(seed)
+    L = np.tril(rng.integers(-3, 4, size=(m, k)).astype(np.float64) / 4.0, -1)
+    for i in range(k):
+        L[i, i] = 1.0
+    U = np.triu(rng.integers(-4, 5, size=(k, n)).astype(np.float64))
+    d = rng.choice([1.0, 2.0, 4.0], size=k) * rng.choice([-1.0, 1.0], size=k)
+    for i in range(k):
+        U[i, i] = d[i]
+    if null_col is not None:
+        U[null_col, null_col] = 0.0
+    return np.asfortranarray(L @ U), np.asfortranarray(L), np.asfortranarray(U)
+
+
+def dominant(n, seed, m=None):
+    """Random U(-1,1) matrix with n added to the diagonal: LU without
+    pivoting is stable on it."""
+    m = n if m is None else m
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1, 1, size=(m, n))
+    for i in range(min(m, n)):
+        A[i, i] += n
+    return np.asfortranarray(A)
+
+
 def perm_from_pivots(p, m):
     """Row permutation (as an index array) produced by the interchanges
     i <-> i + p[i], applied in order i = 0, 1, ...: (P A)[r] = A[perm[r]]."""
