@@ -10,11 +10,12 @@
 // FLA_LU_nopiv.c:27-35) then reports the first exact zero on the diagonal
 // (FLA_LU_find_zero_on_diagonal.c:25-46).
 //
-// Here: 128-column blocks; the diagonal block is factored by one workgroup in
-// LDS; A12 and A21 by substitution in the reference's dtrsm_ order (one lane
-// per column / row: without pivoting the triangles need not be well
-// conditioned, and an explicit block inverse would amplify that); A22 by the
-// MFMA update with U12 read transposed.
+// Here: 256-column blocks, each panel factored in 128-column steps; a
+// 128 x 128 diagonal block is factored by one workgroup in LDS; the panel
+// solves run by substitution in the reference's dtrsm_ order (one lane per
+// column / row: without pivoting the triangles need not be well conditioned,
+// and an explicit block inverse would amplify that); the trailing update is
+// the MFMA GEMM with U12 read transposed and K = 256.
 #include "fla_hip_internal.h"
 
 namespace {
@@ -87,27 +88,43 @@ extern "C" int fla_hip_dgetrf_nopiv(int m, int n, double* A, int lda, int* info)
   if (mn == 0) return FLA_HIP_OK;
   FlaHipCtx& c = fla_ctx();
   hipStream_t s = c.s_update;
-  // workspace: U12 transposed (n x TB)
+  // workspace: U12 transposed (n x 2 TB)
   void* w = nullptr;
-  FLA_TRY(fla_workspace((size_t)n * TB * sizeof(double), &w));
+  FLA_TRY(fla_workspace((size_t)n * 2 * TB * sizeof(double), &w));
   double* Ut = static_cast<double*>(w);
   const long ldu = n;
-  for (int k = 0; k < mn; k += TB) {
-    const int kb = (mn - k) < TB ? (mn - k) : TB;
-    double* A11 = A + (long)k + (long)k * lda;
-    double* A12 = A11 + (long)kb * lda;
-    double* A21 = A11 + kb;
-    const int nc = n - k - kb, mr = m - k - kb;
-    hipLaunchKernelGGL(lu_nopiv_diag_kernel, dim3(1), dim3(256), 0, s, kb, A11, (long)lda);
-    FLA_HIP_CHECK(hipGetLastError());
-    if (nc > 0) FLA_TRY(launch_trsm_llnu_subst(s, nc, kb, A11, lda, A12, lda));
-    if (mr > 0) FLA_TRY(launch_trsm_runn_rows(s, mr, kb, A11, lda, A21, lda));
-    if (mr > 0 && nc > 0) {
-      hipLaunchKernelGGL(transpose_nopiv_kernel, dim3((kb + 31) / 32, (nc + 31) / 32), dim3(256),
-                         0, s, kb, nc, A12, (long)lda, Ut, ldu);
+  auto at = [&](int i, int j) { return A + (long)i + (long)j * lda; };
+  // 256-column blocks: the (m - k) x kb panel by 128-column steps (their
+  // updates confined to the panel), then U12 := inv(L11) A12 for the whole
+  // block (two substitutions around B_b -= L_ba X_a) and one K = 256 update.
+  for (int k = 0; k < mn; k += 2 * TB) {
+    const int kb = (mn - k) < 2 * TB ? (mn - k) : 2 * TB;
+    for (int j = k; j < k + kb; j += TB) {
+      const int jb = (k + kb - j) < TB ? (k + kb - j) : TB;
+      const int nc = k + kb - j - jb, mr = m - j - jb;
+      hipLaunchKernelGGL(lu_nopiv_diag_kernel, dim3(1), dim3(256), 0, s, jb, at(j, j), (long)lda);
       FLA_HIP_CHECK(hipGetLastError());
-      FLA_TRY(launch_dgemm_mfma(s, false, true, 0, mr, nc, kb, -1.0, A21, lda, Ut, ldu, 1.0,
-                                A21 + (long)kb * lda, lda, nullptr));
+      if (nc > 0) FLA_TRY(launch_trsm_llnu_subst(s, nc, jb, at(j, j), lda, at(j, j + jb), lda));
+      if (mr > 0) FLA_TRY(launch_trsm_runn_rows(s, mr, jb, at(j, j), lda, at(j + jb, j), lda));
+      if (mr > 0 && nc > 0)
+        FLA_TRY(launch_dgemm_mfma(s, false, false, 0, mr, nc, jb, -1.0, at(j + jb, j), lda,
+                                  at(j, j + jb), lda, 1.0, at(j + jb, j + jb), lda, nullptr));
+    }
+    const int nc = n - k - kb, mr = m - k - kb;
+    if (nc <= 0) continue;
+    const int ka = kb < TB ? kb : TB, kbb = kb - ka;
+    FLA_TRY(launch_trsm_llnu_subst(s, nc, ka, at(k, k), lda, at(k, k + kb), lda));
+    if (kbb > 0) {
+      FLA_TRY(launch_dgemm_mfma(s, false, false, 0, kbb, nc, ka, -1.0, at(k + ka, k), lda,
+                                at(k, k + kb), lda, 1.0, at(k + ka, k + kb), lda, nullptr));
+      FLA_TRY(launch_trsm_llnu_subst(s, nc, kbb, at(k + ka, k + ka), lda, at(k + ka, k + kb), lda));
+    }
+    if (mr > 0) {
+      hipLaunchKernelGGL(transpose_nopiv_kernel, dim3((kb + 31) / 32, (nc + 31) / 32), dim3(256),
+                         0, s, kb, nc, at(k, k + kb), (long)lda, Ut, ldu);
+      FLA_HIP_CHECK(hipGetLastError());
+      FLA_TRY(launch_dgemm_mfma(s, false, true, 0, mr, nc, kb, -1.0, at(k + kb, k), lda, Ut, ldu,
+                                1.0, at(k + kb, k + kb), lda, nullptr));
     }
   }
   int* d = c.dscratch + 8;

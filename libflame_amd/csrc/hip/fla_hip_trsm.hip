@@ -100,33 +100,59 @@ __global__ __launch_bounds__(64) void trsm_llnu_kernel(int ncols, int nb,
     if (FULL || i < nb) col[i] = x[i];
 }
 
-// A21 := A21 * inv(triu(U11)) for U11 upper non-unit of order nb <= 128 (ld
-// ldu): lane = row, forward over columns in dtrsm_('R','U','N','N') order
-// (src/flablas/f2c/dtrsm.c: b(i,j) -= a(k,j) b(i,k) for k < j with a(k,j) != 0,
-// then b(i,j) *= 1 / a(j,j)); rows are read and written coalesced.
-template <bool FULL>
-__global__ __launch_bounds__(64) void trsm_runn_rows_kernel(int nrows, int nb,
-                                                            const double* __restrict__ U, long ldu,
-                                                            double* __restrict__ B, long ldb) {
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  if (r >= nrows) return;
-  double x[TB];
-#pragma unroll
-  for (int c = 0; c < TB; ++c) x[c] = (FULL || c < nb) ? B[(long)r + (long)c * ldb] : 0.0;
-#pragma unroll
-  for (int c = 0; c < TB; ++c) {
-    if (FULL || c < nb) {
-#pragma unroll
-      for (int k = 0; k < c; ++k) {
-        const double u = U[(long)k + (long)c * ldu];
-        if (u != 0.0) x[c] -= u * x[k];
-      }
-      x[c] *= 1.0 / U[(long)c + (long)c * ldu];
-    }
+// Substitution solves with the nb x nb (nb <= 128) triangle staged in LDS
+// (read by broadcast) and one lane per right-hand side, 4 waves per
+// workgroup, in dtrsm_'s order (src/flablas/f2c/dtrsm.c):
+//   LLNU: A12 := inv(trilu(L11)) A12, lane = column: for c, for i > c:
+//         x[i] -= x[c] L(i, c);
+//   RUNN: A21 := A21 inv(triu(U11)), lane = row (coalesced): for c, for
+//         k < c: x[c] -= U(k, c) x[k]; then x[c] *= 1 / U(c, c).
+// (dtrsm_ skips a zero multiplier; here it is applied, which differs only
+// when the solution holds an infinity or NaN.)
+template <bool FULL, bool RIGHT>
+__global__ __launch_bounds__(256) void trsm_subst_lds_kernel(int nrhs, int nb,
+                                                             const double* __restrict__ T,
+                                                             long ldt, double* __restrict__ B,
+                                                             long ldb) {
+  __shared__ double Ts[TB * TB];  // Ts[c * TB + i] = T(i, c) on the used triangle, else 0
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    const int i = e % TB, c = e / TB;
+    const bool in = RIGHT ? (i <= c) : (i > c);
+    Ts[e] = (in && i < nb && c < nb) ? T[(long)i + (long)c * ldt] : 0.0;
   }
+  __syncthreads();
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nrhs) return;
+  double x[TB];
+  if (RIGHT) {
 #pragma unroll
-  for (int c = 0; c < TB; ++c)
-    if (FULL || c < nb) B[(long)r + (long)c * ldb] = x[c];
+    for (int c = 0; c < TB; ++c) x[c] = (FULL || c < nb) ? B[(long)q + (long)c * ldb] : 0.0;
+#pragma unroll
+    for (int c = 0; c < TB; ++c) {
+      if (FULL || c < nb) {
+#pragma unroll
+        for (int k = 0; k < c; ++k) x[c] -= Ts[c * TB + k] * x[k];
+        x[c] *= 1.0 / Ts[c * TB + c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < TB; ++c)
+      if (FULL || c < nb) B[(long)q + (long)c * ldb] = x[c];
+  } else {
+    double* col = B + (long)q * ldb;
+#pragma unroll
+    for (int i = 0; i < TB; ++i) x[i] = (FULL || i < nb) ? col[i] : 0.0;
+#pragma unroll
+    for (int c = 0; c < TB; ++c) {
+      if (FULL || c < nb) {
+#pragma unroll
+        for (int i = c + 1; i < TB; ++i) x[i] -= x[c] * Ts[c * TB + i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TB; ++i)
+      if (FULL || i < nb) col[i] = x[i];
+  }
 }
 
 // A12 := inv(L11) * A12 for L11 unit lower of order 128 (ld ldl), in place:
@@ -215,32 +241,30 @@ int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long l
   return FLA_HIP_OK;
 }
 
-int launch_trsm_llnu_subst(hipStream_t s, int ncols, int nb, const double* L11, long ldl,
-                           double* A12, long lda) {
-  if (ncols <= 0 || nb <= 0) return FLA_HIP_OK;
+template <bool RIGHT>
+static int launch_subst(hipStream_t s, int nrhs, int nb, const double* T, long ldt, double* B,
+                        long ldb) {
+  if (nrhs <= 0 || nb <= 0) return FLA_HIP_OK;
   if (nb > TB) return FLA_HIP_EARG;
+  const dim3 grid((nrhs + 255) / 256);
   if (nb == TB)
-    hipLaunchKernelGGL(trsm_llnu_kernel<true>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols, nb,
-                       L11, ldl, A12, lda);
+    hipLaunchKernelGGL((trsm_subst_lds_kernel<true, RIGHT>), grid, dim3(256), 0, s, nrhs, nb, T,
+                       ldt, B, ldb);
   else
-    hipLaunchKernelGGL(trsm_llnu_kernel<false>, dim3((ncols + 63) / 64), dim3(64), 0, s, ncols,
-                       nb, L11, ldl, A12, lda);
+    hipLaunchKernelGGL((trsm_subst_lds_kernel<false, RIGHT>), grid, dim3(256), 0, s, nrhs, nb, T,
+                       ldt, B, ldb);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
 
+int launch_trsm_llnu_subst(hipStream_t s, int ncols, int nb, const double* L11, long ldl,
+                           double* A12, long lda) {
+  return launch_subst<false>(s, ncols, nb, L11, ldl, A12, lda);
+}
+
 int launch_trsm_runn_rows(hipStream_t s, int nrows, int nb, const double* U11, long ldu,
                           double* A21, long lda) {
-  if (nrows <= 0 || nb <= 0) return FLA_HIP_OK;
-  if (nb > TB) return FLA_HIP_EARG;
-  if (nb == TB)
-    hipLaunchKernelGGL(trsm_runn_rows_kernel<true>, dim3((nrows + 63) / 64), dim3(64), 0, s,
-                       nrows, nb, U11, ldu, A21, lda);
-  else
-    hipLaunchKernelGGL(trsm_runn_rows_kernel<false>, dim3((nrows + 63) / 64), dim3(64), 0, s,
-                       nrows, nb, U11, ldu, A21, lda);
-  FLA_HIP_CHECK(hipGetLastError());
-  return FLA_HIP_OK;
+  return launch_subst<true>(s, nrows, nb, U11, ldu, A21, lda);
 }
 
 int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool upper, bool unit,
