@@ -773,7 +773,8 @@ static int lu_update_cols2(hipStream_t s, int m, double* A, long lda, int k, int
 // stream carries the chain: panel A; B's columns updated with A; panel B; B's
 // interchanges on A's columns; the next pair's 256 columns updated with the
 // pair (K = 256), after which the next panel A starts.  The bulk stream
-// updates every column beyond the next pair (K = 256) and applies the pair's
+// (s_update_lp: its CU mask leaves 64 CUs to the panel kernel) updates every
+// column beyond the next pair (K = 256) and applies the pair's
 // interchanges left of it, concurrently with the next pair's two panels; the
 // chain waits for it only before the following pair's columns.  Ut holds the
 // transposed U12 of the bulk update in rows [0, n) and of the chain's updates
@@ -781,7 +782,7 @@ static int lu_update_cols2(hipStream_t s, int m, double* A, long lda, int k, int
 static int lu_blocked2(FlaHipCtx& c, int m, int n, double* A, long lda, int* p, int* dinfo,
                        LuBufs (&bufs)[4], double* Ut, long ldu) {
   const int mn = m < n ? m : n;
-  hipStream_t su = c.s_update, sb = c.s_bulk, sq = c.s_panel;
+  hipStream_t su = c.s_update, sb = c.s_update_lp, sq = c.s_panel;
   double* Utn = Ut + n;
   int evi = 0;
   auto next_event = [&]() {

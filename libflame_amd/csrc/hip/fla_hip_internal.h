@@ -34,8 +34,9 @@ struct FlaHipCtx {
   hipStream_t s_panel = nullptr;   // panel factorizations (look-ahead)
   // Trailing updates of the reduced-precision factor: their panel chain, not
   // the update, is the critical path, so this stream leaves more CUs free.
+  // 64 CUs left to the panels: the reduced-precision factor's updates and the
+  // LU's bulk updates beside its panel chain
   hipStream_t s_update_lp = nullptr;
-  hipStream_t s_bulk = nullptr;  // LU: bulk trailing updates beside the panel pair's chain
   static constexpr int kEvents = 8;
   hipEvent_t ev[kEvents] = {};
   // Grow-only device workspaces: slot 0 general scratch (dtrsm, dlaswp),

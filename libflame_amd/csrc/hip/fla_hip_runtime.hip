@@ -3,9 +3,10 @@
 //
 // The reference binds one rocBLAS handle per SuperMatrix thread and device
 // (FLASH_Queue_bind_hip, src/base/flamec/supermatrix/hip/main/FLASH_Queue_hip.c:297-318);
-// here one process drives one GPU through a few streams: s_update for trailing
-// updates, s_panel for the look-ahead panel chain, s_bulk for the LU's bulk
-// updates beside them, s_update_lp for the reduced-precision factor.
+// here one process drives one GPU through three streams: s_update for trailing
+// updates, s_panel for the look-ahead panel chain, and s_update_lp (64 CUs
+// left to the panels) for the reduced-precision factor's updates and the
+// LU's bulk updates.
 #include <cstring>
 
 #include "fla_hip_internal.h"
@@ -21,14 +22,16 @@ FlaHipCtx& fla_ctx() { return g_ctx; }
 // single-workgroup panel kernel that needs a whole CU's LDS would otherwise
 // wait for a CU to drain completely.
 static int g_reserved_cus = 8;
-// The reduced-precision factor's update stream leaves 64 CUs to its panels
-// (measured on MI355X at n = 16384).
+// The second update stream leaves 64 CUs to the panels.  It carries the
+// reduced-precision factor's updates (measured on MI355X at n = 16384) and
+// the LU's bulk updates, whose panel kernel needs up to 64 whole CUs with all
+// its workgroups resident (with 8 left, LU panels waited for CUs: 119 ms
+// instead of 87 ms per factorization at n = 16384).  One stream serves both
+// so that the library's three streams, with the null stream, fit the four
+// hardware queues HIP gives a process: two busy streams sharing a queue
+// serialize (an LU with its bulk stream on the panel's queue took 182 ms
+// instead of 134 ms).
 static int g_lp_reserved_cus = 64;
-// The LU's bulk-update stream leaves 64 CUs to the panel kernel, whose up to
-// 64 workgroups each need a whole CU and must all be resident (measured at
-// n = 16384: with 8 left, panels waited for CUs and took 119 ms instead of
-// 87 ms per factorization).
-static int g_bulk_reserved_cus = 64;
 
 static int create_masked_stream(hipStream_t* s, int reserved) {
   if (reserved <= 0 || reserved >= g_ctx.num_cus) {
@@ -78,7 +81,6 @@ extern "C" int fla_hip_init(void) {
   // Default-priority streams only.
   FLA_TRY(create_update_stream(g_reserved_cus));
   FLA_TRY(create_masked_stream(&g_ctx.s_update_lp, g_lp_reserved_cus));
-  FLA_TRY(create_masked_stream(&g_ctx.s_bulk, g_bulk_reserved_cus));
   FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_panel, hipStreamNonBlocking));
   for (int i = 0; i < FlaHipCtx::kEvents; ++i)
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
@@ -111,7 +113,6 @@ extern "C" int fla_hip_finalize(void) {
   for (int i = 0; i < FlaHipCtx::kEvents; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev[i]));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update_lp));
-  FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_bulk));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_panel));
   g_ctx.ready = false;
   return FLA_HIP_OK;
@@ -121,7 +122,6 @@ extern "C" int fla_hip_synchronize(void) {
   FLA_TRY(fla_ensure_ready());
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_panel));
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update_lp));
-  FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_bulk));
   FLA_HIP_CHECK(hipStreamSynchronize(g_ctx.s_update));
   return FLA_HIP_OK;
 }
