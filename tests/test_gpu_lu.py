@@ -98,3 +98,31 @@ def test_apply_pivots_and_shift(orc):
     assert np.array_equal(dp.download().ravel(), p + np.arange(m) + 1)
     assert fla.FLA_Shift_pivots_to(fla.FLA_NATIVE_PIVOTS, dp.obj()) == fla.FLA_SUCCESS
     assert np.array_equal(dp.download().ravel(), p)
+
+
+@pytest.mark.parametrize("m", [16641, 33000])
+def test_lu_tall_panel_many_workgroups(orc, m):
+    """A panel taller than 64 x 256 rows: more than 64 workgroups, so every
+    header poll and the pivot selection run over several rounds of the
+    wave's lanes.  Exact closed form (pivots, L, U)."""
+    n = 192
+    A, L, U, p = exact_lu(n, seed=m, m=m)
+    out, pv, r = gpu_lu(A)
+    assert r == fla.FLA_SUCCESS
+    assert np.array_equal(pv, p)
+    assert np.array_equal(np.tril(out[:, :n], -1), np.tril(L, -1))
+    assert np.array_equal(np.triu(out[:n, :]), U)
+
+
+@pytest.mark.parametrize("rows", [[16700], [5, 16900], [16999]])
+def test_lu_tall_panel_nan_rule(orc, rows):
+    """The NaN restart rule across more than 64 workgroups: pivots of the
+    first columns identical to the oracle's."""
+    m, n = 17000, 130
+    A = np.asfortranarray(orc.gen_lu_c2(9, m, n))
+    for r in rows:
+        A[r, 0] = np.nan
+    _, pv, _ = gpu_lu(A)
+    _, pref, _ = orc.lu_piv(A)
+    assert pv[0] == pref[0]
+    assert pv[0] >= max(rows)  # the scan restarts after the last NaN
