@@ -480,40 +480,51 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     const double* u = swap ? prow : jrow;
     const int jn = j + 1;
     if (cg >= cgj) {
+      double l[RQ], xn[RQ] = {0.0, 0.0};
+      bool act[RQ];
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
         const int lr = rh * 128 + 64 * q + lane;
         const int rr = base + lr;
-        const bool act = rr > j && rr < M;
+        act[q] = rr > j && rr < M;
         double x = colbuf[par][lr];
         if (swap && rr == r) x = jrow[j];
-        const double l = scale ? x * inv : x;
-        double xn = 0.0;
-        // 8-column chunks [c0, c0 + 8): wholly right of j without selects;
-        // the chunk holding j (only in the waves owning it) with them
+        l[q] = scale ? x * inv : x;
+      }
+      // 8-column chunks [c0, c0 + 8): wholly right of j without selects; the
+      // chunk holding j (only in the waves owning it) with them.  The chunk's
+      // pivot-row values are read once for both rows of the thread.
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-          const int c0 = 32 * cg + 8 * cc;
-          if (c0 + 7 < j) continue;  // uniform
+      for (int cc = 0; cc < 4; ++cc) {
+        const int c0 = 32 * cg + 8 * cc;
+        if (c0 + 7 < j) continue;  // uniform
+        double uu[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) uu[c] = u[c0 + c];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
           double* a = &v[q][8 * cc];
           if (c0 > j) {
-            if (act) {
+            if (act[q]) {
 #pragma unroll
-              for (int c = 0; c < 8; ++c) a[c] = fma(-l, u[c0 + c], a[c]);
+              for (int c = 0; c < 8; ++c) a[c] = fma(-l[q], uu[c], a[c]);
             }
-          } else if (act) {
+          } else if (act[q]) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-              const double y = fma(-l, u[c0 + c], a[c]);
-              a[c] = (c0 + c > j) ? y : ((c0 + c == j) ? l : a[c]);
+              const double y = fma(-l[q], uu[c], a[c]);
+              a[c] = (c0 + c > j) ? y : ((c0 + c == j) ? l[q] : a[c]);
             }
           }
           if (jn < W && c0 <= jn && jn < c0 + 8) {  // uniform: the next column's chunk
 #pragma unroll
-            for (int c = 0; c < 8; ++c) xn = (c0 + c == jn) ? a[c] : xn;
+            for (int c = 0; c < 8; ++c) xn[q] = (c0 + c == jn) ? a[c] : xn[q];
           }
         }
-        if (jn < W && cg == (jn >> 5)) colbuf[par ^ 1][lr] = xn;
+      }
+      if (jn < W && cg == (jn >> 5)) {
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) colbuf[par ^ 1][rh * 128 + 64 * q + lane] = xn[q];
       }
     }
   }
