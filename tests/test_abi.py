@@ -131,3 +131,16 @@ def test_inverse_lapack_argument_errors(capfd):
     L.dpotri_(C.c_char_p(b"U"), ip(n), ap, ip(bad), ip(info))
     assert info[0] == -4
     assert "DTRTRI" in capfd.readouterr().err
+
+
+def test_lu_blocksize_values():
+    """lu_nb: multiples of 32 up to 128 (one panel per step) or 256 (pairs of
+    128-column panels); anything else is an argument error.  Host-only."""
+    import libflame_amd as fla
+
+    L = fla.lib()
+    for ok in (32, 64, 96, 128, 256):
+        assert L.fla_hip_set_blocksizes(0, ok) == 0
+    for bad in (16, 160, 192, 224, 288, 512):
+        assert L.fla_hip_set_blocksizes(0, bad) != 0
+    assert L.fla_hip_set_blocksizes(0, 256) == 0
