@@ -23,6 +23,8 @@
  *   FLA_Trsm/Herk/Syrk/Gemm_external  src/base/flamec/wrappers/blas/3/external/
  *   dpotrf_ / dgetrf_          src/map/lapack2flamec/FLA_potrf.c:31-105, FLA_getrf.c:32-117
  *   dpotrs_ / dgetrs_          src/map/common/lapacksrc/f2c/dpotrs.c:102, dgetrs.c:115
+ *   dlaswp_                    src/map/common/lapacksrc/f2c/dlaswp.c:103
+ *   dposv_ / dgesv_            src/map/common/lapacksrc/f2c/dposv.c:120, dgesv.c:113
  *   dsposv_                    src/map/common/lapacksrc/f2c/dsposv.c:192-415
  *   dsgesv_                    src/map/common/lapacksrc/f2c/dsgesv.c:189-407
  */
@@ -250,6 +252,9 @@ int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* 
             int* info);
 int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b,
             int* ldb, int* info);
+int dlaswp_(int* n, double* a, int* lda, int* k1, int* k2, int* ipiv, int* incx);
+int dposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, int* info);
+int dgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb, int* info);
 int dsposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, double* x,
             int* ldx, double* work, float* swork, int* iter, int* info);
 int dsgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb, double* x,

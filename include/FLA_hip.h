@@ -96,9 +96,33 @@ int fla_hip_dspdinv(int uplo, int n, double* A, int lda, int* info);
    FLAME-native pivots (0-based, relative to the row).  *info = FLA_SUCCESS or
    the 0-based index of the first null pivot. */
 int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* info);
-/* Row interchanges i <-> i + p[i] for i = k1..k2 (in that order, or reversed
-   when incp < 0) on the ncols columns of A.  p is a device pointer. */
+/* Row interchanges i <-> i + p[i * |incp|] for i = k1..k2 (in that order, or
+   reversed when incp < 0) on the ncols columns of A; p is a device
+   pointer to FLAME-native (0-based, relative) pivots.  The final source of
+   every touched row is resolved on the device and only touched rows move;
+   asynchronous on the library's stream.  Row indices are bounded by lda; an
+   interchange whose target row lies outside [0, lda) is skipped. */
 int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, const int* p, int incp);
+/* The same with the rows of the m x ncols matrix: targets outside [0, m) are
+   skipped (FLA_Apply_pivots on an m-row object). */
+int fla_hip_dlaswp_rows(int m, int ncols, double* A, int lda, int k1, int k2, const int* p,
+                        int incp);
+/* ---- LAPACK auxiliaries on device matrices ----
+   norm: '1' / 'O' (max column sum), 'I' (max row sum), 'M' (max |a_ij|),
+   'F' / 'E' (Frobenius); *value receives the norm (synchronizes).  NaN
+   propagates.  Replace dlange_ / dlansy_ (src/map/common/lapacksrc/f2c/
+   dlange.c, dlansy.c) for device buffers; dlansy reads only the uplo
+   triangle of the symmetric matrix. */
+int fla_hip_dlange(int norm, int m, int n, const double* A, int lda, double* value);
+int fla_hip_dlansy(int norm, int uplo, int n, const double* A, int lda, double* value);
+/* B := A on the uplo trapezoid (FLA_LOWER_TRIANGULAR: i >= j,
+   FLA_UPPER_TRIANGULAR: i <= j, anything else: all of A); dlacpy_
+   (src/map/common/lapacksrc/f2c/dlacpy.c).  Asynchronous. */
+int fla_hip_dlacpy(int uplo, int m, int n, const double* A, int lda, double* B, int ldb);
+/* Off-diagonal entries of the uplo part (strictly lower / strictly upper /
+   all) := alpha, diagonal := beta; dlaset_ (src/map/common/lapacksrc/f2c/
+   dlaset.c).  Asynchronous. */
+int fla_hip_dlaset(int uplo, int m, int n, double alpha, double beta, double* A, int lda);
 /* p[i] += (i+1) (to LAPACK) or -= (i+1) (to FLAME) on a device int vector. */
 int fla_hip_shift_pivots(int ptype, int m, int* p);
 

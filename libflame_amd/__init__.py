@@ -91,6 +91,9 @@ def lib():
             "dgetrf_": (C.c_int, [ip, ip, dp, ip, ip, ip]),
             "dpotrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, ip]),
             "dgetrs_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, ip, dp, ip, ip]),
+            "dlaswp_": (C.c_int, [ip, dp, ip, ip, ip, ip, ip]),
+            "dposv_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, ip]),
+            "dgesv_": (C.c_int, [ip, ip, dp, ip, ip, dp, ip, ip]),
             "dsposv_": (C.c_int, [C.c_char_p, ip, ip, dp, ip, dp, ip, dp, ip, dp,
                                   C.POINTER(C.c_float), ip, ip]),
             "dsgesv_": (C.c_int, [ip, ip, dp, ip, ip, dp, ip, dp, ip, dp,
@@ -128,7 +131,14 @@ def lib():
             "fla_hip_dspdinv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
             "fla_hip_dgetrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, ip]),
             "fla_hip_dlaswp": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_dlaswp_rows": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int, vp,
+                                              C.c_int]),
             "fla_hip_shift_pivots": (C.c_int, [C.c_int, C.c_int, vp]),
+            "fla_hip_dlange": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, dp]),
+            "fla_hip_dlansy": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, dp]),
+            "fla_hip_dlacpy": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int]),
+            "fla_hip_dlaset": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, vp,
+                                         C.c_int]),
             "fla_hip_stats_enable": (C.c_int, [C.c_int]),
             "fla_hip_stats_reset": (C.c_int, []),
             "fla_hip_stats_get": (C.c_int, [C.c_int, dp, dp, C.POINTER(C.c_long)]),

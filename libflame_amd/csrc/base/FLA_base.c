@@ -103,8 +103,13 @@ void FLA_Init_safe(FLA_Error* init_result) {
   }
 }
 
+/* The *_safe pair brackets one library call made without FLA_Init (the
+   LAPACK entry points): it sets up and clears only the FLAME-level state.
+   The device context (streams, events, workspaces) is kept for the next
+   call and released by FLA_Finalize or at process exit, so a LAPACK caller
+   does not recreate streams on every call. */
 void FLA_Finalize_safe(FLA_Error init_result) {
-  if (init_result == FLA_SUCCESS) FLA_Finalize();
+  if (init_result == FLA_SUCCESS) fla_initialized = 0;
 }
 
 unsigned int FLA_Check_error_level(void) { return fla_error_level; }

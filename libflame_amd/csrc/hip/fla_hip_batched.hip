@@ -704,6 +704,25 @@ extern "C" int fla_hip_dgesv_batched(int count, const int* n, double* const* A, 
                        reinterpret_cast<void* const*>(B), ldb, nrhs, info);
 }
 
+// Releases the cached graph and the item buffers (fla_hip_finalize; the
+// device must be idle).  Everything is recreated on the next batched call.
+int fla_batched_release() {
+  int st = FLA_HIP_OK;
+  if (g_graph.exec && hipGraphExecDestroy(g_graph.exec) != hipSuccess) st = FLA_HIP_EARG;
+  if (g_graph.graph && hipGraphDestroy(g_graph.graph) != hipSuccess) st = FLA_HIP_EARG;
+  g_graph.exec = nullptr;
+  g_graph.graph = nullptr;
+  g_graph.valid = false;
+  if (g_items_dev && hipFree(g_items_dev) != hipSuccess) st = FLA_HIP_EARG;
+  if (g_items_pin && hipHostFree(g_items_pin) != hipSuccess) st = FLA_HIP_EARG;
+  if (g_counter && hipFree(g_counter) != hipSuccess) st = FLA_HIP_EARG;
+  g_items_dev = nullptr;
+  g_items_pin = nullptr;
+  g_counter = nullptr;
+  g_items_cap = 0;
+  return st;
+}
+
 extern "C" int fla_hip_batched_set_graph(int on) {
   g_use_graph = on != 0;
   return FLA_HIP_OK;

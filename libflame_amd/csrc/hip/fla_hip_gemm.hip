@@ -7,10 +7,11 @@
 //     src/lapack/dec/lu/piv/vars/flamec/FLA_LU_piv_blk_var5.c:94-96 -> dgemm_('N','N')
 //   * the FLA_Gemm / FLA_Syrk entry points of the thin C ABI.
 //
-// Tiling: a 256-thread workgroup (4 wave64s in a 2x2 grid) owns a 128x128 tile
-// of C; each wave owns 64x64 = 4x4 blocks of v_mfma_f64_16x16x4_f64.  K is
-// streamed in chunks of 16 through two LDS buffers (load chunk t+1 into
-// registers while chunk t feeds the MFMAs, then one barrier per chunk).
+// Tiling (default configuration): a 512-thread workgroup (8 wave64s in a 4x2
+// grid) owns a 128x128 tile of C; each wave owns 32x64 = 2x4 blocks of
+// v_mfma_f64_16x16x4_f64.  K is streamed in chunks of 16 (8 when B is
+// k-contiguous) through two LDS buffers (load chunk t+1 into registers while
+// chunk t feeds the MFMAs, then one barrier per chunk).
 // The MFMA is issued with the B fragment as its "A" operand so that each
 // lane's four accumulators are four columns of one row index: for fixed
 // accumulator slot, lanes 0..15 hold 16 consecutive rows of one column, and
@@ -244,7 +245,11 @@ extern "C" int fla_hip_dgemm(int transa, int transb, int m, int n, int k, double
   if (ldc < m || lda < (ta ? k : m) || ldb < (tb ? n : k)) return FLA_HIP_EARG;
   if (k == 0 || alpha == 0.0) {
     if (beta == 1.0) return FLA_HIP_OK;
-    // k == 0 still scales C: run with k = 0 (loop skipped, epilogue applies beta).
+    // C := beta*C without reading A or B (src/flablas/f2c/dgemm.c:194-214):
+    // run with k = 0 (the K loop is skipped, the epilogue applies beta and
+    // writes 0 where beta == 0).
+    k = 0;
+    alpha = 0.0;
   }
   return launch_dgemm_mfma(fla_ctx().s_update, ta, tb, 0, m, n, k, alpha, A, lda, B, ldb,
                            beta, C, ldc, nullptr);
@@ -258,7 +263,11 @@ extern "C" int fla_hip_dsyrk(int uplo, int trans, int n, int k, double alpha,
   if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
   const bool t = is_trans(trans);
   if (ldc < n || lda < (t ? k : n)) return FLA_HIP_EARG;
-  if ((k == 0 || alpha == 0.0) && beta == 1.0) return FLA_HIP_OK;
+  if (k == 0 || alpha == 0.0) {
+    if (beta == 1.0) return FLA_HIP_OK;
+    k = 0;  // C := beta*C on the triangle without reading A (dsyrk.c's quick return)
+    alpha = 0.0;
+  }
   hipStream_t s = fla_ctx().s_update;
   if (uplo == kLower) {
     // C(lower) += alpha * op(A) op(A)'   with op(A) n x k.

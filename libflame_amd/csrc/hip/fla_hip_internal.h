@@ -91,6 +91,11 @@ int fla_workspace_slot(int slot, size_t bytes, void** ptr);
     if (s_ != FLA_HIP_OK) return s_; \
   } while (0)
 
+// Releases the batched path's cached graph and item buffers, and the
+// measurement event pool (called by fla_hip_finalize with the device idle).
+int fla_batched_release();
+void fla_stats_release();
+
 // ---- measurement hooks (no-ops unless fla_hip_stats_enable(1)) -------------
 bool fla_stats_on();
 void* fla_stats_begin(hipStream_t s);
@@ -218,5 +223,8 @@ int fla_lu_panel_run(hipStream_t s, int set, int M, int W, double* Ap, long lda,
 // first row is the panel's first row (FP64 / FP32).
 int fla_lu_gather_rows(hipStream_t s, int set, int ncols, double* A, long lda);
 int fla_lu_gather_rows_f(hipStream_t s, int set, int ncols, float* A, long lda);
+// Row interchanges of fla_hip_dlaswp on stream s, rows bounded by `rows`.
+int launch_laswp(hipStream_t s, int rows, int ncols, double* A, long lda, int k1, int k2,
+                 const int* p, int incp);
 // In-place exchange of the strictly lower and strictly upper triangles.
 int launch_transpose_swap(hipStream_t s, int n, double* A, long lda);

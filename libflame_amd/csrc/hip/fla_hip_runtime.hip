@@ -7,6 +7,7 @@
 // updates, s_panel for the look-ahead panel chain, and s_update_lp (64 CUs
 // left to the panels) for the reduced-precision factor's updates and the
 // LU's bulk updates.
+#include <cstdlib>
 #include <cstring>
 
 #include "fla_hip_internal.h"
@@ -62,6 +63,14 @@ int fla_ensure_ready() {
   return fla_hip_init();
 }
 
+// Process exit with the library's streams, events and graph still alive
+// leaves their destruction to the HIP runtime's own teardown, after tools
+// such as the profiler have already shut down.  The library therefore
+// releases them itself from an exit handler registered after the runtime
+// initialised (handlers run in reverse order of registration, so this one
+// runs while the runtime is still whole).
+static void fla_release_at_exit() { (void)fla_hip_finalize(); }
+
 extern "C" int fla_hip_init(void) {
   if (g_ctx.ready) return FLA_HIP_OK;
   int ndev = 0;
@@ -95,12 +104,19 @@ extern "C" int fla_hip_init(void) {
   }
   g_ctx.uc = uc;
   g_ctx.ready = true;
+  static bool exit_hook = false;
+  if (!exit_hook) {
+    exit_hook = true;
+    std::atexit(fla_release_at_exit);
+  }
   return FLA_HIP_OK;
 }
 
 extern "C" int fla_hip_finalize(void) {
   if (!g_ctx.ready) return FLA_HIP_OK;
   FLA_HIP_CHECK(hipDeviceSynchronize());
+  fla_stats_release();
+  FLA_TRY(fla_batched_release());
   for (int i = 0; i < FlaHipCtx::kSlots; ++i) {
     if (g_ctx.work[i]) FLA_HIP_CHECK(hipFree(g_ctx.work[i]));
     g_ctx.work[i] = nullptr;
@@ -329,6 +345,16 @@ void fla_stats_harvest() {
     g_stats.pool.push_back(p.stop);
   }
   g_stats.pending.clear();
+}
+
+void fla_stats_release() {
+  for (auto& p : g_stats.pending) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  g_stats.pending.clear();
+  for (hipEvent_t e : g_stats.pool) (void)hipEventDestroy(e);
+  g_stats.pool.clear();
 }
 
 extern "C" int fla_hip_stats_enable(int on) {

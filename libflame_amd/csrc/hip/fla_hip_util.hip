@@ -57,16 +57,96 @@ __global__ void shift_pivots_kernel(int to_lapack, int m, int* p) {
   if (i < m) p[i] += to_lapack ? (i + 1) : -(i + 1);
 }
 
-// dst(q, :) := src(perm[q], :), perm[q] being the source row of final row q
-// of the interchanges i <-> i + p[i*incp] (fla_hip_dlaswp).
-__global__ void gather_rows_kernel(int rows, int ncols, const int* __restrict__ perm,
-                                   const double* __restrict__ src, long lds,
-                                   double* __restrict__ dst, long ldd) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)rows * ncols;
-  if (idx >= total) return;
-  const int q = (int)(idx % rows), j = (int)(idx / rows);
-  dst[(long)q + (long)j * ldd] = src[(long)perm[q] + (long)j * lds];
+// ---- row interchanges (FLA_Apply_pivots / dlaswp) on the device ----------
+// The interchange sequence: step q = 0..K-1 (in application order) swaps rows
+// r(q) and t(q); forward order r(q) = k1 + q, reverse order r(q) = k2 - q, and
+// t(q) = r(q) + p[(r(q) - k1) * stride] (p points at row k1's pivot, so the
+// caller's vector holds row i's pivot at i * stride, as
+// FLA_Apply_pivots_ln_opt_var1.c:313-330 reads it).  A step
+// whose t lies outside [0, rows) is skipped, so no row outside the matrix is
+// ever addressed.
+struct SwapSeq {
+  const int* p;
+  int k1, k2, stride, rev, rows;
+  __device__ __forceinline__ void step(int q, int& r, int& t) const {
+    r = rev ? k2 - q : k1 + q;
+    t = r + p[(long)(r - k1) * stride];
+    if (t < 0 || t >= rows) t = r;
+  }
+};
+
+constexpr int kSwapChunk = 1024;
+
+// Final source row of every touched row, without walking the sequence on one
+// thread: slot q < K resolves row r(q); slot K + q resolves t(q) when it lies
+// outside [k1, k2] (rows inside are resolved by their own slot).  Each slot
+// follows its row backwards through the whole sequence (x = t at a step whose
+// r is x, x = r at a step whose t is x); the steps are staged through LDS in
+// chunks and read as LDS broadcasts.  pos[slot] = -1 marks an unused slot.
+__global__ __launch_bounds__(256) void laswp_resolve_kernel(SwapSeq S, int K, int* __restrict__ pos,
+                                                            int* __restrict__ src) {
+  __shared__ int sr[kSwapChunk], st[kSwapChunk];
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  int x0 = -1;
+  if (slot < 2 * K) {
+    const int q = slot < K ? slot : slot - K;
+    int r, t;
+    S.step(q, r, t);
+    if (slot < K)
+      x0 = r;
+    else
+      x0 = (t == r || (t >= S.k1 && t <= S.k2)) ? -1 : t;
+  }
+  int x = x0;
+  for (int c0 = ((K - 1) / kSwapChunk) * kSwapChunk; c0 >= 0; c0 -= kSwapChunk) {
+    const int cn = min(kSwapChunk, K - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cn; e += 256) {
+      int r, t;
+      S.step(c0 + e, r, t);
+      sr[e] = r;
+      st[e] = t;
+    }
+    __syncthreads();
+    if (x0 >= 0) {
+      for (int e = cn - 1; e >= 0; --e) {
+        const int r = sr[e], t = st[e];
+        x = (x == r) ? t : ((x == t) ? r : x);
+      }
+    }
+  }
+  if (slot < 2 * K) {
+    pos[slot] = x0;
+    src[slot] = x;
+  }
+}
+
+// tmp(v, j) := A(src[v], j) for every slot whose row changes.
+__global__ __launch_bounds__(256) void laswp_gather_kernel(int nslot, int ncols,
+                                                           const int* __restrict__ pos,
+                                                           const int* __restrict__ src,
+                                                           const double* __restrict__ A, long lda,
+                                                           double* __restrict__ tmp) {
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= nslot) return;
+  const int x = pos[v], y = src[v];
+  if (x < 0 || x == y) return;
+  for (int j = blockIdx.y; j < ncols; j += gridDim.y)
+    tmp[(long)v + (long)j * nslot] = A[(long)y + (long)j * lda];
+}
+
+// A(pos[v], j) := tmp(v, j).  Two slots naming one row carry identical data.
+__global__ __launch_bounds__(256) void laswp_scatter_kernel(int nslot, int ncols,
+                                                            const int* __restrict__ pos,
+                                                            const int* __restrict__ src,
+                                                            const double* __restrict__ tmp,
+                                                            double* __restrict__ A, long lda) {
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= nslot) return;
+  const int x = pos[v], y = src[v];
+  if (x < 0 || x == y) return;
+  for (int j = blockIdx.y; j < ncols; j += gridDim.y)
+    A[(long)x + (long)j * lda] = tmp[(long)v + (long)j * nslot];
 }
 
 // ---- FP64 reference products for residual checks -----------------------
@@ -297,51 +377,55 @@ extern "C" int fla_hip_shift_pivots(int ptype, int m, int* p) {
   return fla_hip_synchronize();
 }
 
+// Row interchanges i <-> i + p[i * |incp|] for i = k1..k2 (reverse
+// order for incp < 0) on the ncols columns of A, rows limited to [0, rows):
+// the final source of each touched row is resolved on the device and only
+// those rows move (gather into a workspace, then scatter), asynchronously on
+// stream s.  Replaces the walk of FLA_Apply_pivots_ln_opt_var1
+// (src/lapack/util/app/piv/ln/flamec/FLA_Apply_pivots_ln_opt_var1.c:313-330).
+int launch_laswp(hipStream_t s, int rows, int ncols, double* A, long lda, int k1, int k2,
+                 const int* p, int incp) {
+  if (ncols <= 0 || k2 < k1 || incp == 0 || rows <= 0) return FLA_HIP_OK;
+  if (k1 < 0 || k2 >= rows) return FLA_HIP_EARG;
+  const int K = k2 - k1 + 1;
+  if (K > (1 << 29)) return FLA_HIP_EARG;
+  const int nslot = 2 * K;
+  const size_t idx_bytes = ((size_t)nslot * sizeof(int) + 255) & ~(size_t)255;
+  void* w = nullptr;
+  FLA_TRY(fla_workspace(2 * idx_bytes + (size_t)nslot * ncols * sizeof(double), &w));
+  int* pos = static_cast<int*>(w);
+  int* src = reinterpret_cast<int*>(static_cast<char*>(w) + idx_bytes);
+  double* tmp = reinterpret_cast<double*>(static_cast<char*>(w) + 2 * idx_bytes);
+  SwapSeq S;
+  S.p = p + (long)k1 * (incp > 0 ? incp : -incp);
+  S.k1 = k1;
+  S.k2 = k2;
+  S.stride = incp > 0 ? incp : -incp;
+  S.rev = incp < 0;
+  S.rows = rows;
+  const unsigned gs = (unsigned)((nslot + 255) / 256);
+  hipLaunchKernelGGL(laswp_resolve_kernel, dim3(gs), dim3(256), 0, s, S, K, pos, src);
+  const unsigned gy = (unsigned)(ncols < 64 ? ncols : 64);
+  hipLaunchKernelGGL(laswp_gather_kernel, dim3(gs, gy), dim3(256), 0, s, nslot, ncols, pos, src,
+                     A, lda, tmp);
+  hipLaunchKernelGGL(laswp_scatter_kernel, dim3(gs, gy), dim3(256), 0, s, nslot, ncols, pos, src,
+                     tmp, A, lda);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
 extern "C" int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, const int* p,
                               int incp) {
   FLA_TRY(fla_ensure_ready());
-  if (ncols <= 0 || k2 < k1 || incp == 0) return FLA_HIP_OK;
-  hipStream_t s = fla_ctx().s_update;
-  // The interchanges touch rows up to max(i + p[i]); the permutation is built
-  // over rows [0, lda) (every row index must lie inside the leading dimension).
-  const int rows = lda;
-  // The interchange sequence is inherently sequential: walk it on the host
-  // (one small copy each way) rather than in a single GPU thread.
-  const int pstride = incp > 0 ? incp : -incp;
-  const long plen = (long)(k2 - k1) * pstride + 1;
-  std::vector<int> hp((size_t)plen), perm((size_t)rows);
-  FLA_HIP_CHECK(hipMemcpyAsync(hp.data(), p + (long)k1 * pstride, sizeof(int) * plen,
-                               hipMemcpyDeviceToHost, s));
-  FLA_HIP_CHECK(hipStreamSynchronize(s));
-  for (int q = 0; q < rows; ++q) perm[q] = q;
-  const int step = incp > 0 ? 1 : -1;
-  const int ib = incp > 0 ? k1 : k2;
-  const int ie = incp > 0 ? k2 + 1 : k1 - 1;
-  for (int i = ib; i != ie; i += step) {
-    if (i < 0 || i >= rows) break;
-    const int t = i + hp[(size_t)(i - k1) * pstride];
-    if (t < 0 || t >= rows) break;  // pivot outside the matrix: leave the rest alone
-    const int a = perm[i];
-    perm[i] = perm[t];
-    perm[t] = a;
-  }
-  void* w = nullptr;
-  const size_t perm_bytes = ((size_t)rows * sizeof(int) + 255) & ~(size_t)255;
-  FLA_TRY(fla_workspace(perm_bytes + (size_t)rows * ncols * sizeof(double), &w));
-  int* dperm = static_cast<int*>(w);
-  double* tmp = reinterpret_cast<double*>(static_cast<char*>(w) + perm_bytes);
-  FLA_HIP_CHECK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * rows, hipMemcpyHostToDevice, s));
-  // tmp := A (rows x ncols), then A(q, :) := tmp(perm[q], :)
-  FLA_HIP_CHECK(hipMemcpy2DAsync(tmp, (size_t)rows * sizeof(double), A, (size_t)lda * sizeof(double),
-                                 (size_t)rows * sizeof(double), (size_t)ncols,
-                                 hipMemcpyDeviceToDevice, s));
-  const long total = (long)rows * ncols;
-  hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     rows, ncols, dperm, tmp, (long)rows, A, (long)lda);
-  FLA_HIP_CHECK(hipGetLastError());
-  // perm (host) must outlive the upload
-  FLA_HIP_CHECK(hipStreamSynchronize(s));
-  return FLA_HIP_OK;
+  // Every row index must lie inside the leading dimension.
+  return launch_laswp(fla_ctx().s_update, lda, ncols, A, lda, k1, k2, p, incp);
+}
+
+extern "C" int fla_hip_dlaswp_rows(int m, int ncols, double* A, int lda, int k1, int k2,
+                                   const int* p, int incp) {
+  FLA_TRY(fla_ensure_ready());
+  if (m > lda) return FLA_HIP_EARG;
+  return launch_laswp(fla_ctx().s_update, m, ncols, A, lda, k1, k2, p, incp);
 }
 
 // Symmetric positive definite A = Q diag(s) Q^T, s_i = kappa^(i/(n-1)),
@@ -434,6 +518,216 @@ extern "C" int fla_hip_convert_d2s(int m, int n, const double* A, int lda, float
   const long total = (long)m * n;
   hipLaunchKernelGGL(d2s_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      fla_ctx().s_update, m, n, A, (long)lda, S, (long)lds);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+// ---- LAPACK auxiliaries on device matrices: dlange / dlansy norms, dlacpy,
+// dlaset (netlib semantics: src/map/common/lapacksrc/f2c/dlange.c,
+// dlansy.c, dlacpy.c, dlaset.c).  The norms reduce per column (or per row
+// for the row sums of a stored triangle) and then over columns; NaN
+// propagates.
+namespace {
+
+// Row range of column j that a reduction covers: 0 = all m rows,
+// 1 = rows j..m-1 (lower triangle with the diagonal), 2 = rows 0..j (upper).
+__device__ __forceinline__ void col_range(int mode, int m, int j, int& r0, int& r1) {
+  r0 = mode == 1 ? j : 0;
+  r1 = mode == 2 ? (j + 1 < m ? j + 1 : m) : m;
+}
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  return (b > a || b != b) ? b : a;
+}
+
+// op 0: sum |a|, 1: max |a|, 2: sum a^2, over the covered rows of column j.
+__global__ __launch_bounds__(256) void col_reduce_kernel(int op, int mode, int m, int n,
+                                                         const double* __restrict__ A, long lda,
+                                                         double* __restrict__ out) {
+  const int j = blockIdx.x;
+  if (j >= n) return;
+  int r0, r1;
+  col_range(mode, m, j, r0, r1);
+  double acc = 0.0;
+  for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) {
+    const double a = A[(long)i + (long)j * lda];
+    acc = op == 0 ? acc + fabs(a) : (op == 1 ? nanmax(acc, fabs(a)) : acc + a * a);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double b = red[threadIdx.x + s];
+      red[threadIdx.x] = op == 1 ? nanmax(red[threadIdx.x], b) : red[threadIdx.x] + b;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// Sum of |A(i, c)| over the columns c of row i: mode 0 all n columns,
+// 1 columns 0..i-1 (strictly lower part of the row), 2 columns i+1..n-1.
+__global__ __launch_bounds__(256) void row_abssum_kernel(int mode, int m, int n,
+                                                         const double* __restrict__ A, long lda,
+                                                         double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const int c0 = mode == 2 ? i + 1 : 0;
+  const int c1 = mode == 1 ? (i < n ? i : n) : n;
+  double acc = 0.0;
+  for (int c = c0; c < c1; ++c) acc += fabs(A[(long)i + (long)c * lda]);
+  out[i] = acc;
+}
+
+// out[0] = op over k < cnt of (a[k] + (b ? b[k] : 0)): op 1 max, 0 sum.
+__global__ __launch_bounds__(256) void final_reduce_kernel(int op, int cnt,
+                                                           const double* __restrict__ a,
+                                                           const double* __restrict__ b,
+                                                           double* __restrict__ out) {
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < cnt; k += 256) {
+    const double v = a[k] + (b ? b[k] : 0.0);
+    acc = op == 1 ? nanmax(acc, v) : acc + v;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double c = red[threadIdx.x + s];
+      red[threadIdx.x] = op == 1 ? nanmax(red[threadIdx.x], c) : red[threadIdx.x] + c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// mode 0 full, 1 lower trapezoid (i >= j), 2 upper trapezoid (i <= j).
+__global__ void lacpy_kernel(int mode, int m, int n, const double* __restrict__ A, long lda,
+                             double* __restrict__ B, long ldb) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  if ((mode == 1 && i < j) || (mode == 2 && i > j)) return;
+  B[(long)i + (long)j * ldb] = A[(long)i + (long)j * lda];
+}
+
+// dlaset: off-diagonal entries of the selected part := alpha, diagonal := beta.
+__global__ void laset_kernel(int mode, int m, int n, double alpha, double beta,
+                             double* __restrict__ A, long lda) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  if (i == j)
+    A[(long)i + (long)j * lda] = beta;
+  else if (mode == 0 || (mode == 1 && i > j) || (mode == 2 && i < j))
+    A[(long)i + (long)j * lda] = alpha;
+}
+
+int norm_kind(int norm) {
+  switch (norm) {
+    case '1': case 'O': case 'o': return 1;
+    case 'I': case 'i': return 2;
+    case 'M': case 'm': return 3;
+    case 'F': case 'f': case 'E': case 'e': return 4;
+    default: return 0;
+  }
+}
+
+int uplo_mode(int uplo) { return uplo == kLower ? 1 : (uplo == kUpper ? 2 : 0); }
+
+// Shared driver of dlange / dlansy: m x n, sym = 0 general, else the
+// symmetric matrix stored in the uplo triangle (m == n).
+int norm_driver(int kind, int sym_mode, int m, int n, const double* A, long lda, double* value) {
+  hipStream_t s = fla_ctx().s_update;
+  void* w = nullptr;
+  const size_t len = (size_t)(m > n ? m : n);
+  FLA_TRY(fla_workspace_slot(3, (2 * len + 8) * sizeof(double), &w));
+  double* colv = static_cast<double*>(w);
+  double* rowv = colv + len;
+  double* res = rowv + len;
+  const bool onecol = (kind == 1) || (sym_mode != 0 && kind == 2);  // symmetric: ||.||_1 == ||.||_inf
+  if (onecol) {
+    hipLaunchKernelGGL(col_reduce_kernel, dim3(n), dim3(256), 0, s, 0, sym_mode, m, n, A, lda,
+                       colv);
+    if (sym_mode != 0)
+      hipLaunchKernelGGL(row_abssum_kernel, dim3((m + 255) / 256), dim3(256), 0, s, sym_mode, m,
+                         n, A, lda, rowv);
+    hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(256), 0, s, 1, n, colv,
+                       sym_mode != 0 ? rowv : nullptr, res);
+  } else if (kind == 2) {
+    hipLaunchKernelGGL(row_abssum_kernel, dim3((m + 255) / 256), dim3(256), 0, s, 0, m, n, A, lda,
+                       rowv);
+    hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(256), 0, s, 1, m, rowv, nullptr, res);
+  } else if (kind == 3) {
+    hipLaunchKernelGGL(col_reduce_kernel, dim3(n), dim3(256), 0, s, 1, sym_mode, m, n, A, lda,
+                       colv);
+    hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(256), 0, s, 1, n, colv, nullptr, res);
+  } else {
+    // Frobenius: a symmetric matrix counts each stored off-diagonal entry twice
+    hipLaunchKernelGGL(col_reduce_kernel, dim3(n), dim3(256), 0, s, 2, sym_mode, m, n, A, lda,
+                       colv);
+    hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(256), 0, s, 0, n, colv, nullptr, res);
+    if (sym_mode != 0) {
+      hipLaunchKernelGGL(col_reduce_kernel, dim3(n), dim3(256), 0, s, 2, 0, 1, n, A, lda + 1,
+                         rowv);  // the diagonal: a 1 x n matrix with stride lda + 1
+      hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(256), 0, s, 0, n, rowv, nullptr,
+                         res + 1);
+    }
+  }
+  FLA_HIP_CHECK(hipGetLastError());
+  double h[2] = {0.0, 0.0};
+  FLA_HIP_CHECK(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  FLA_HIP_CHECK(hipStreamSynchronize(s));
+  if (kind == 4) *value = sym_mode != 0 ? sqrt(2.0 * h[0] - h[1]) : sqrt(h[0]);
+  else *value = h[0];
+  return FLA_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int fla_hip_dlange(int norm, int m, int n, const double* A, int lda, double* value) {
+  FLA_TRY(fla_ensure_ready());
+  const int kind = norm_kind(norm);
+  if (!value || kind == 0 || m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
+  *value = 0.0;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  return norm_driver(kind, 0, m, n, A, lda, value);
+}
+
+extern "C" int fla_hip_dlansy(int norm, int uplo, int n, const double* A, int lda,
+                              double* value) {
+  FLA_TRY(fla_ensure_ready());
+  const int kind = norm_kind(norm);
+  if (!value || kind == 0 || (uplo != kLower && uplo != kUpper) || n < 0 ||
+      lda < (n > 1 ? n : 1))
+    return FLA_HIP_EARG;
+  *value = 0.0;
+  if (n == 0) return FLA_HIP_OK;
+  return norm_driver(kind, uplo_mode(uplo), n, n, A, lda, value);
+}
+
+extern "C" int fla_hip_dlacpy(int uplo, int m, int n, const double* A, int lda, double* B,
+                              int ldb) {
+  FLA_TRY(fla_ensure_ready());
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || ldb < (m > 1 ? m : 1)) return FLA_HIP_EARG;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(lacpy_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     fla_ctx().s_update, uplo_mode(uplo), m, n, A, (long)lda, B, (long)ldb);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dlaset(int uplo, int m, int n, double alpha, double beta, double* A,
+                              int lda) {
+  FLA_TRY(fla_ensure_ready());
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(laset_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     fla_ctx().s_update, uplo_mode(uplo), m, n, alpha, beta, A, (long)lda);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

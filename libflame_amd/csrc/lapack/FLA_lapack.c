@@ -323,8 +323,9 @@ FLA_Error FLA_Apply_pivots(FLA_Side side, FLA_Trans trans, FLA_Obj p, FLA_Obj A)
     fla_devmat_release(&dA, 0);
     return e;
   }
-  st = fla_hip_dlaswp(dA.n, (double*)dA.dev, dA.ld, 0, (int)FLA_Obj_vector_dim(p) - 1,
-                      (const int*)dp.dev, 1);
+  st = fla_hip_dlaswp_rows(dA.m, dA.n, (double*)dA.dev, dA.ld, 0,
+                           (int)FLA_Obj_vector_dim(p) - 1, (const int*)dp.dev, 1);
+  if (st == FLA_HIP_OK) st = fla_hip_synchronize();
   e = fla_devmat_release(&dA, st == FLA_HIP_OK);
   fla_devmat_release(&dp, 0);
   if (st != FLA_HIP_OK) return gpu_status(st, "FLA_Apply_pivots");
@@ -594,8 +595,8 @@ FLA_Error FLA_LU_piv_solve(FLA_Obj A, FLA_Obj p, FLA_Obj B, FLA_Obj X) {
     fla_devmat_release(&dX, 0);
     return e;
   }
-  st = fla_hip_dlaswp(dX.n, (double*)dX.dev, dX.ld, 0, (int)FLA_Obj_vector_dim(p) - 1,
-                      (const int*)dp.dev, 1);
+  st = fla_hip_dlaswp_rows(dX.m, dX.n, (double*)dX.dev, dX.ld, 0,
+                           (int)FLA_Obj_vector_dim(p) - 1, (const int*)dp.dev, 1);
   if (st == FLA_HIP_OK)
     st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_UNIT_DIAG, dX.m,
                        dX.n, 1.0, (const double*)dA.dev, dA.ld, (double*)dX.dev, dX.ld);

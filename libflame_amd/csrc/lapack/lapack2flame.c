@@ -8,6 +8,9 @@
  *                       FLA_Shift_pivots_to(FLA_LAPACK_PIVOTS))
  *   dpotrs_            src/map/common/lapacksrc/f2c/dpotrs.c:102-194
  *   dgetrs_            src/map/common/lapacksrc/f2c/dgetrs.c:115-210
+ *   dlaswp_            src/map/common/lapacksrc/f2c/dlaswp.c:103-213
+ *   dposv_ / dgesv_    src/map/common/lapacksrc/f2c/dposv.c:120-190,
+ *                      dgesv.c:113-180 (factor, then solve when info == 0)
  *   dtrtri_ / dlauum_ / dpotri_
  *                      src/map/lapack2flamec/FLA_trtri.c, FLA_lauum.c,
  *                      FLA_potri.c (FLA_Trinv / FLA_Ttmm on the factor)
@@ -217,6 +220,151 @@ int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, doub
     fprintf(stderr, "libflame_amd: dgetrs_ failed: %s\n", fla_hip_error_string(st));
     *info = -100;
   }
+  return 0;
+}
+
+/* dlaswp_ (src/map/common/lapacksrc/f2c/dlaswp.c:103-213): interchange row i
+   with row ipiv(ix) for i = k1..k2 (incx > 0, ix = k1 + (i-k1)*incx) or for
+   i = k2..k1 (incx < 0, ix = 1 + (i-1)*|incx|); incx == 0 does nothing.  The
+   rows that the interchanges reach are staged to the device, permuted there
+   by the device interchange kernels, and copied back. */
+int dlaswp_(int* n, double* a, int* lda, int* k1, int* k2, int* ipiv, int* incx) {
+  int i, st, rows, K, ncols = *n;
+  int* rel = NULL;
+  int* dp = NULL;
+  double* dA = NULL;
+  if (*incx == 0 || ncols <= 0 || *k2 < *k1 || *k1 < 1) return 0;
+  K = *k2; /* the device vector holds row r's pivot at r (rows 0..k2-1) */
+  rel = (int*)calloc((size_t)K, sizeof(int));
+  if (!rel) return 0;
+  /* FLAME-native (0-based, relative) pivot of each row, and the rows the
+     interchanges reach */
+  rows = *k2;
+  for (i = *k1; i <= *k2; ++i) {
+    long ix = *incx > 0 ? (long)*k1 + (long)(i - *k1) * *incx : 1 + (long)(i - 1) * -(long)*incx;
+    int ip = ipiv[ix - 1];
+    rel[i - 1] = ip - i;
+    if (ip > rows) rows = ip;
+  }
+  if (rows > *lda) rows = *lda; /* the device kernels skip targets beyond the rows */
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)rows * (size_t)ncols);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dp, sizeof(int) * (size_t)K);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(rows, ncols, a, *lda, dA, rows);
+  if (st == FLA_HIP_OK) st = fla_hip_memcpy_h2d(dp, rel, sizeof(int) * (size_t)K);
+  if (st == FLA_HIP_OK)
+    st = fla_hip_dlaswp_rows(rows, ncols, dA, rows, *k1 - 1, *k2 - 1, dp, *incx > 0 ? 1 : -1);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(rows, ncols, dA, rows, a, *lda);
+  fla_hip_free(dA);
+  fla_hip_free(dp);
+  free(rel);
+  if (st != FLA_HIP_OK) fprintf(stderr, "libflame_amd: dlaswp_ failed: %s\n", fla_hip_error_string(st));
+  return 0;
+}
+
+/* dposv_ (src/map/common/lapacksrc/f2c/dposv.c:120-190): the netlib checks,
+   then the Cholesky factorization (dpotrf_'s semantics, info = first failing
+   column + 1) and, when it succeeded, the two triangular solves -- with A and
+   B staged to the device once for both. */
+int dposv_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, int* info) {
+  int u = up(*uplo), st, e = FLA_SUCCESS, ul;
+  double *dA = NULL, *dB = NULL;
+  *info = 0;
+  if (u != 'U' && u != 'L')
+    *info = -1;
+  else if (*n < 0)
+    *info = -2;
+  else if (*nrhs < 0)
+    *info = -3;
+  else if (*lda < imax(1, *n))
+    *info = -5;
+  else if (*ldb < imax(1, *n))
+    *info = -7;
+  if (*info != 0) {
+    xerbla("DPOSV ", -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  ul = u == 'U' ? FLA_UPPER_TRIANGULAR : FLA_LOWER_TRIANGULAR;
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)*n * (size_t)*n);
+  if (st == FLA_HIP_OK && *nrhs > 0)
+    st = fla_hip_malloc((void**)&dB, sizeof(double) * (size_t)*n * (size_t)*nrhs);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *n, a, *lda, dA, *n);
+  if (st == FLA_HIP_OK) st = fla_hip_dpotrf(ul, *n, dA, *n, &e);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(*n, *n, dA, *n, a, *lda);
+  if (st == FLA_HIP_OK && e == FLA_SUCCESS && *nrhs > 0) {
+    st = fla_hip_dmatrix_h2d(*n, *nrhs, b, *ldb, dB, *n);
+    if (st == FLA_HIP_OK)
+      st = fla_hip_dtrsm(FLA_LEFT, ul, ul == FLA_UPPER_TRIANGULAR ? FLA_TRANSPOSE : FLA_NO_TRANSPOSE,
+                         FLA_NONUNIT_DIAG, *n, *nrhs, 1.0, dA, *n, dB, *n);
+    if (st == FLA_HIP_OK)
+      st = fla_hip_dtrsm(FLA_LEFT, ul, ul == FLA_UPPER_TRIANGULAR ? FLA_NO_TRANSPOSE : FLA_TRANSPOSE,
+                         FLA_NONUNIT_DIAG, *n, *nrhs, 1.0, dA, *n, dB, *n);
+    if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(*n, *nrhs, dB, *n, b, *ldb);
+  }
+  fla_hip_free(dA);
+  fla_hip_free(dB);
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: dposv_ failed: %s\n", fla_hip_error_string(st));
+    *info = -100;
+    return 0;
+  }
+  *info = (e != FLA_SUCCESS) ? e + 1 : 0;
+  return 0;
+}
+
+/* dgesv_ (src/map/common/lapacksrc/f2c/dgesv.c:113-180): the netlib checks,
+   LU with partial pivoting (dgetrf_'s semantics: LAPACK pivots in ipiv,
+   info = first null pivot + 1) and, when no pivot was null, the pivoted
+   unit-lower and upper solves -- A, B and the pivots staying on the device
+   between the two steps. */
+int dgesv_(int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb, int* info) {
+  int st, e = FLA_SUCCESS, i;
+  double *dA = NULL, *dB = NULL;
+  int* dp = NULL;
+  *info = 0;
+  if (*n < 0)
+    *info = -1;
+  else if (*nrhs < 0)
+    *info = -2;
+  else if (*lda < imax(1, *n))
+    *info = -4;
+  else if (*ldb < imax(1, *n))
+    *info = -7;
+  if (*info != 0) {
+    xerbla("DGESV ", -*info);
+    return 0;
+  }
+  if (*n == 0) return 0;
+  st = fla_hip_malloc((void**)&dA, sizeof(double) * (size_t)*n * (size_t)*n);
+  if (st == FLA_HIP_OK && *nrhs > 0)
+    st = fla_hip_malloc((void**)&dB, sizeof(double) * (size_t)*n * (size_t)*nrhs);
+  if (st == FLA_HIP_OK) st = fla_hip_malloc((void**)&dp, sizeof(int) * (size_t)*n);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_h2d(*n, *n, a, *lda, dA, *n);
+  if (st == FLA_HIP_OK) st = fla_hip_dgetrf(*n, *n, dA, *n, dp, &e);
+  if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(*n, *n, dA, *n, a, *lda);
+  if (st == FLA_HIP_OK) st = fla_hip_memcpy_d2h(ipiv, dp, sizeof(int) * (size_t)*n);
+  if (st == FLA_HIP_OK && e == FLA_SUCCESS && *nrhs > 0) {
+    st = fla_hip_dmatrix_h2d(*n, *nrhs, b, *ldb, dB, *n);
+    if (st == FLA_HIP_OK) st = fla_hip_dlaswp(*nrhs, dB, *n, 0, *n - 1, dp, 1);
+    if (st == FLA_HIP_OK)
+      st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_UNIT_DIAG, *n,
+                         *nrhs, 1.0, dA, *n, dB, *n);
+    if (st == FLA_HIP_OK)
+      st = fla_hip_dtrsm(FLA_LEFT, FLA_UPPER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_NONUNIT_DIAG, *n,
+                         *nrhs, 1.0, dA, *n, dB, *n);
+    if (st == FLA_HIP_OK) st = fla_hip_dmatrix_d2h(*n, *nrhs, dB, *n, b, *ldb);
+  }
+  fla_hip_free(dA);
+  fla_hip_free(dB);
+  fla_hip_free(dp);
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: dgesv_ failed: %s\n", fla_hip_error_string(st));
+    *info = -100;
+    return 0;
+  }
+  /* FLAME-native (0-based relative) -> LAPACK (1-based absolute) */
+  for (i = 0; i < *n; ++i) ipiv[i] += i + 1;
+  *info = (e != FLA_SUCCESS) ? e + 1 : 0;
   return 0;
 }
 

@@ -1011,7 +1011,8 @@ double or_chol_residual(int uplo, int n, const double* a0, int lda0, const doubl
         if (s > rmax) rmax = s;
     }
     free(col);
-    if (anorm <= 0.0) return rmax == 0.0 ? 0.0 : INFINITY;
+    /* dpot01.f:151-156: RESID = 1/EPS when ANORM <= 0 */
+    if (anorm <= 0.0) return 1.0 / eps;
     return rmax / ((double)n * eps * anorm);
 }
 
@@ -1043,6 +1044,8 @@ double or_lu_residual(int m, int n, const double* a0, int lda0, const double* f,
         if (r > rmax) rmax = r;
     }
     free(pa);
-    if (anorm <= 0.0) return rmax == 0.0 ? 0.0 : INFINITY;
-    return rmax / ((double)imin(m, n) * eps * anorm);
+    /* dget01.f:209-214: RESID / N / ANORM / EPS; 1/EPS when ANORM <= 0 and
+       the difference is not zero */
+    if (anorm <= 0.0) return rmax == 0.0 ? 0.0 : 1.0 / eps;
+    return rmax / ((double)n * eps * anorm);
 }

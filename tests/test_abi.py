@@ -144,3 +144,40 @@ def test_lu_blocksize_values():
     for bad in (16, 160, 192, 224, 288, 512):
         assert L.fla_hip_set_blocksizes(0, bad) != 0
     assert L.fla_hip_set_blocksizes(0, 256) == 0
+
+
+def test_driver_lapack_argument_errors(capfd):
+    """dposv_ / dgesv_ argument checks (dposv.c:153-175, dgesv.c:145-166:
+    -1 uplo/n, ... -7 ldb) return before any device work; dlaswp_ with
+    incx == 0 or k2 < k1 does nothing (dlaswp.c:130-145)."""
+    import ctypes as C
+
+    import numpy as np
+
+    import libflame_amd as fla
+
+    L = fla.lib()
+    a = np.arange(16.0).reshape(4, 4, order="F")
+    b = np.ones((4, 2), order="F")
+    ap = a.ctypes.data_as(C.POINTER(C.c_double))
+    bp = b.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    v = lambda x: np.array([x], np.int32)  # noqa: E731
+    info = v(0)
+    piv = np.array([4, 3, 3, 4], np.int32)
+    for args, want in [((b"X", 4, 2, 4, 4), -1), ((b"L", -1, 2, 4, 4), -2),
+                       ((b"L", 4, -1, 4, 4), -3), ((b"L", 4, 2, 3, 4), -5),
+                       ((b"L", 4, 2, 4, 3), -7)]:
+        u, n, nr, lda, ldb = args
+        L.dposv_(C.c_char_p(u), ip(v(n)), ip(v(nr)), ap, ip(v(lda)), bp, ip(v(ldb)), ip(info))
+        assert info[0] == want
+    for args, want in [((-1, 2, 4, 4), -1), ((4, -1, 4, 4), -2), ((4, 2, 3, 4), -4),
+                       ((4, 2, 4, 3), -7)]:
+        n, nr, lda, ldb = args
+        L.dgesv_(ip(v(n)), ip(v(nr)), ap, ip(v(lda)), ip(piv), bp, ip(v(ldb)), ip(info))
+        assert info[0] == want
+    before = a.copy()
+    L.dlaswp_(ip(v(4)), ap, ip(v(4)), ip(v(1)), ip(v(4)), ip(piv), ip(v(0)))
+    L.dlaswp_(ip(v(4)), ap, ip(v(4)), ip(v(3)), ip(v(2)), ip(piv), ip(v(1)))
+    assert np.array_equal(a, before) and np.array_equal(b, np.ones((4, 2)))
+    assert "DPOSV" in capfd.readouterr().err

@@ -81,16 +81,20 @@ def test_dscal_bitwise_vs_reference(orc):
         assert x.tobytes() == y.tobytes()
 
 
-@pytest.mark.parametrize("incx", [1, -1])
-def test_dlaswp_vs_reference(orc, incx):
+@pytest.mark.parametrize("incx", [1, -1, 2, -2, 3])
+@pytest.mark.parametrize("forward_only", [True, False])
+def test_dlaswp_vs_reference(orc, incx, forward_only):
+    """The strided forms index ipiv as dlaswp.c:130-145 does (incx > 0 from
+    k1, incx < 0 from 1); ipiv may point above the row as well."""
     ref = _ref_or_skip(orc)
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(11 + incx)
     m, n = 70, 45
     A = np.asfortranarray(rng.standard_normal((m, n)))
     k1, k2 = 3, 60
-    ipiv = np.zeros(k2, dtype=np.int32)
-    for i in range(k2):
-        ipiv[i] = rng.integers(i + 1, m + 1)  # 1-based, >= i
+    ipiv = np.zeros(k2 * abs(incx) + 1, dtype=np.int32)
+    for i in range(len(ipiv)):
+        lo = min(i, m - 1) + 1 if forward_only else 1
+        ipiv[i] = rng.integers(lo, m + 1)  # 1-based
     A1 = A.copy(order="F")
     A2 = A.copy(order="F")
     orc.lib().or_dlaswp(n, orc.dp(A1), m, k1, k2, orc.ip(ipiv), incx)

@@ -226,3 +226,138 @@ def test_generators_deterministic_and_distributed(orc):
     g = orc.gen_normal(5, 400, 300)
     assert abs(g.mean()) < 0.01 and abs(g.var() - 1.0) < 0.02
     assert not np.array_equal(orc.gen_uniform(6, 10, 10), orc.gen_uniform(5, 10, 10))
+
+
+# ---- pins of the oracle's error metrics and of or_dpotrs --------------------
+# The residual ratios are the dpot01 / dget01 forms of the LAPACK test suite
+# (netlib-test/lapack-3.4.2/TESTING/LIN/dpot01.f:88,151-156,213-217 and
+# dget01.f:154-214): ||L L' - A||_1 / (N eps ||A||_1) with eps = 2^-53.  They
+# are pinned against an independent numpy evaluation on factors carrying a
+# known dyadic perturbation, where every product and sum is exact in binary64,
+# so the two must agree to the rounding of the final division.  A dropped N,
+# eps = 2^-52, the infinity norm, a one-sided triangle of the difference or a
+# read of the unreferenced triangle changes the value.
+
+_EPS = 2.0 ** -53
+
+
+def _norm1(M):
+    return float(np.abs(M).sum(axis=0).max())
+
+
+def _perturb(F, seed, tri, count=7, scale=2.0 ** -16):
+    rng = np.random.default_rng(seed)
+    G = F.copy(order="F")
+    n0, n1 = G.shape
+    placed = 0
+    while placed < count:
+        i, j = int(rng.integers(0, n0)), int(rng.integers(0, n1))
+        if tri == "lower" and i < j or tri == "upper" and i > j:
+            continue
+        G[i, j] += scale * int(rng.integers(1, 8)) * (1 if rng.random() < 0.5 else -1)
+        placed += 1
+    return G
+
+
+@pytest.mark.parametrize("n", [5, 33, 130])
+@pytest.mark.parametrize("upper", [False, True])
+def test_chol_residual_metric_pinned(orc, n, upper):
+    A, F = exact_spd(n, seed=100 + n, upper=upper)
+    uplo = orc.UPPER if upper else orc.LOWER
+    # the exact factor: the difference is exactly zero
+    assert orc.chol_residual(A, F, uplo=uplo) == 0.0
+    G = _perturb(F, seed=n, tri="upper" if upper else "lower")
+    R = (G.T @ G if upper else G @ G.T) - A  # exact: integers plus multiples of 2^-32
+    want = _norm1(R) / (n * _EPS * _norm1(A))
+    # the unreferenced triangles of A0 and of the factor must not be read
+    A_dirty, G_dirty = A.copy(order="F"), G.copy(order="F")
+    if upper:
+        A_dirty[np.tril_indices(n, -1)] = np.nan
+        G_dirty[np.tril_indices(n, -1)] = np.nan
+    else:
+        A_dirty[np.triu_indices(n, 1)] = np.nan
+        G_dirty[np.triu_indices(n, 1)] = np.nan
+    got = orc.chol_residual(A_dirty, G_dirty, uplo=uplo)
+    assert want > 0 and abs(got - want) <= 1e-12 * want
+
+
+def test_chol_residual_zero_matrix(orc):
+    # dpot01.f:151-156: 1/eps when ||A|| = 0
+    Z = np.zeros((4, 4), order="F")
+    assert orc.chol_residual(Z, Z) == 1.0 / _EPS
+
+
+@pytest.mark.parametrize("shape", [(6, 6), (70, 70), (90, 40), (40, 90)])
+def test_lu_residual_metric_pinned(orc, shape):
+    m, n = shape
+    k = min(m, n)
+    A, L, U, p = exact_lu(n, seed=m * 7 + n, m=m)
+    F = np.zeros((m, n), order="F")
+    F[:, :k] += np.tril(L, -1)
+    F[:k, :] += U
+    assert orc.lu_residual(A, F, p) == 0.0
+    G = _perturb(F, seed=m + n, tri="none")
+    L2 = np.tril(G[:, :k], -1)
+    L2[:k, :k] += np.eye(k)
+    U2 = np.triu(G[:k, :])
+    PA = A[perm_from_pivots(p, m), :]
+    R = L2 @ U2 - PA
+    # dget01 divides by N, the number of columns (dget01.f:209-214)
+    want = _norm1(R) / (n * _EPS * _norm1(A))
+    got = orc.lu_residual(A, G, p)
+    assert want > 0 and abs(got - want) <= 1e-12 * want
+    # the infinity norm of the same difference differs (the pin tells them apart)
+    assert abs(float(np.abs(R).sum(axis=1).max()) / (n * _EPS * _norm1(A)) - want) > 1e-6 * want
+
+
+def test_lu_residual_zero_matrix(orc):
+    Z = np.zeros((5, 5), order="F")
+    assert orc.lu_residual(Z, Z, np.zeros(5, np.int32)) == 0.0
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("n", [1, 40, 161])
+def test_dpotrs_exact_dyadic(orc, n, upper):
+    """or_dpotrs (dpotrs.c:179-189): with the exact dyadic factor F of A and
+    integer X, every intermediate of both substitutions is an integer, so
+    B = A X is solved back to X bit for bit; solving the two triangles in the
+    wrong order or with the wrong transpose does not return X."""
+    import ctypes as C
+    nrhs = 9
+    A, F = exact_spd(n, seed=300 + n, upper=upper)
+    rng = np.random.default_rng(n)
+    X = np.asfortranarray(rng.integers(-4, 5, size=(n, nrhs)).astype(np.float64))
+    B = np.asfortranarray(A @ X)
+    Fd = F.copy(order="F")
+    # the other triangle holds garbage: dpotrs references only uplo's
+    if upper:
+        Fd[np.tril_indices(n, -1)] = np.nan
+    else:
+        Fd[np.triu_indices(n, 1)] = np.nan
+    info = C.c_int(7)
+    orc.lib().or_dpotrs(b"U" if upper else b"L", n, nrhs, orc.dp(Fd), n, orc.dp(B), n,
+                        C.byref(info))
+    assert info.value == 0
+    assert np.array_equal(B, X)
+    if n > 1:
+        # The swapped order of the two substitutions (a plausible mistake)
+        # solves F'F X = B instead of F F' X = B and does not return X.
+        ul = "U" if upper else "L"
+        first, second = ("N", "T") if upper else ("T", "N")
+        Bw = orc.dtrsm("L", ul, first, "N", F, A @ X)
+        Bw = orc.dtrsm("L", ul, second, "N", F, Bw)
+        assert not np.array_equal(Bw, X)
+
+
+def test_dpotrs_argument_checks(orc):
+    import ctypes as C
+    a = np.eye(4, order="F")
+    b = np.ones((4, 2), order="F")
+    info = C.c_int(0)
+    L = orc.lib()
+    for args, want in [((b"X", 4, 2, 4, 4), -1), ((b"L", -1, 2, 4, 4), -2), ((b"L", 4, -1, 4, 4), -3),
+                       ((b"L", 4, 2, 3, 4), -5), ((b"L", 4, 2, 4, 3), -7)]:
+        u, n, nrhs, lda, ldb = args
+        L.or_dpotrs(u, n, nrhs, orc.dp(a), lda, orc.dp(b), ldb, C.byref(info))
+        assert info.value == want
+    assert np.array_equal(b, np.ones((4, 2)))
