@@ -88,24 +88,80 @@ def chol_probe_residual(fla, np, A0, L, n, nprobe=4):
                      for j in range(nprobe)))
 
 
-def cpu_baseline(fla, n_cpu):
-    """The oracle's FLA_Chol (libflame's control tree, f2c loop order, one
-    thread) on a bounded n_cpu sample of the same matrix family, generated on
-    the GPU (A = B*B'/n + I) and copied to the host."""
+def _oracle_omp():
+    """The oracle's OpenMP build (oracle/liboracle_omp.so): libflame's control
+    trees with dgemm/dsyrk/dtrsm split over threads by independent columns or
+    rows, bit-identical to the serial oracle."""
+    import ctypes as C
+    path = os.path.join(ROOT, "oracle", "liboracle_omp.so")
+    L = C.CDLL(path)
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.or_threads.restype = C.c_int
+    L.or_chol.restype = C.c_int
+    L.or_chol.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, C.c_int]
+    L.or_lu_piv.restype = C.c_int
+    L.or_lu_piv.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int]
+    return L
+
+
+def cpu_baseline(fla, n_chol, n_lu):
+    """The oracle's FLA_Chol and FLA_LU_piv (libflame's control trees: var3
+    128/32 + opt_var2, var5 128/16 + Crout opt_var4; f2c BLAS loop order) on
+    every host core the process may use (the OpenMP build, identical results
+    to the serial oracle), on bounded samples of the same matrix families
+    generated on the GPU and copied to the host."""
+    import ctypes as C
+
     import numpy as np
-    from oracle import oracle as orc
-    dA = make_spd(fla, n_cpu, 11)
+    L = _oracle_omp()
+    cores = int(L.or_threads())
+    dA = make_spd(fla, n_chol, 11)
     A = dA.download()
     A = np.asfortranarray(np.tril(A) + np.tril(A, -1).T)
     del dA
+    nb = np.array([128, 32], np.int32)
     t0 = time.perf_counter()
-    _, r = orc.chol(A)
+    r = L.or_chol(300, n_chol, A.ctypes.data_as(C.POINTER(C.c_double)), n_chol,
+                  nb.ctypes.data_as(C.POINTER(C.c_int)), 2)
     dt = time.perf_counter() - t0
-    assert r == orc.SUCCESS
-    return {"value": round(n_cpu ** 3 / 3.0 / dt / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+    assert r == -1
+    log(f"cpu baseline: FLA_Chol n={n_chol} {dt:.1f} s on {cores} threads")
+    del A
+    dL = make_lu(fla, n_lu, 7)
+    M = np.asfortranarray(dL.download())
+    del dL
+    p = np.zeros(n_lu, np.int32)
+    nbl = np.array([128, 16], np.int32)
+    t1 = time.perf_counter()
+    r = L.or_lu_piv(n_lu, n_lu, M.ctypes.data_as(C.POINTER(C.c_double)), n_lu,
+                    p.ctypes.data_as(C.POINTER(C.c_int)), nbl.ctypes.data_as(C.POINTER(C.c_int)), 2)
+    dtl = time.perf_counter() - t1
+    assert r == -1
+    log(f"cpu baseline: FLA_LU_piv n={n_lu} {dtl:.1f} s on {cores} threads")
+    return {"value": round(n_chol ** 3 / 3.0 / dt / 1e12, 6), "unit": "TFLOP/s", "cores": cores,
             "kind": "port",
-            "sample": f"oracle FLA_Chol lower (var3 128/32 + opt_var2, f2c BLAS order), "
-                      f"n={n_cpu}, A=B*B'/n+I, {dt:.1f} s on one host core"}
+            "lu_value": round(2.0 * n_lu ** 3 / 3.0 / dtl / 1e12, 6),
+            "sample": f"oracle FLA_Chol lower (var3 128/32 + opt_var2, f2c BLAS order) n={n_chol}, "
+                      f"A=B*B'/n+I: {dt:.1f} s; oracle FLA_LU_piv (var5 128/16 + Crout "
+                      f"opt_var4) n={n_lu}, U[-1,1] rows scaled 10^(-i/n): {dtl:.1f} s (lu_value); "
+                      f"OpenMP build on {cores} threads, results identical to the serial oracle"}
+
+
+def lu_moved_bytes(np, piv, n, W=128):
+    """HBM bytes of the LU's row interchanges from its pivot vector: per
+    128-column panel, every row the panel's interchanges move is read and
+    written once in each of the n - W columns outside the panel."""
+    total = 0
+    for k in range(0, n, W):
+        w = min(W, n - k)
+        perm = {}
+        for j in range(k, k + w):
+            t = j + int(piv[j])
+            a, b = perm.get(j, j), perm.get(t, t)
+            perm[j], perm[t] = b, a
+        moved = sum(1 for r, src in perm.items() if r != src)
+        total += moved * (n - w) * 16
+    return total
 
 
 def bench_mixed(fla, np, n, nrhs=8):
@@ -490,7 +546,8 @@ def main():
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--chol-nb", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=5120)
+    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--cpu-lu-n", type=int, default=6144)
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -590,11 +647,27 @@ def main():
         fla.stats_enable(False)
         gf, gms, gn = fla.stats_get(fla.STATS_LU_UPDATE)
         pf, pms, pn = fla.stats_get(fla.STATS_LU_PANEL)
+        sf, sms, sn = fla.stats_get(fla.STATS_LU_SWAP)
         t_lu = sum(lt) / len(lt)
+        nrep = len(lt)
+        swap_bytes = lu_moved_bytes(np, piv.download().ravel(), m)
+        # panel kernel: the (m-k) x 128 panel read once and written once
+        panel_bytes = sum(2.0 * 8 * (m - k) * min(128, m - k) for k in range(0, m, 128))
+        upd_tf = gf / (gms * 1e-3) / 1e12 if gms else 0.0
         extras.update({"lu_n": m, "lu_tflops": round(2 * m ** 3 / 3 / t_lu / 1e12, 3),
                        "lu_ms": round(t_lu * 1e3, 2),
-                       "lu_update_kernel_tflops": round(gf / (gms * 1e-3) / 1e12, 3) if gms else None,
-                       "lu_panel_ms_per_factorization": round(pms / len(lt), 2)})
+                       "lu_update_kernel_tflops": round(upd_tf, 3),
+                       "lu_panel_ms_per_factorization": round(pms / nrep, 2),
+                       "lu_panel_GBps": round(panel_bytes / (pms / nrep * 1e-3) / 1e9, 1) if pms else None,
+                       "lu_swap_ms_per_factorization": round(sms / nrep, 2),
+                       "lu_swap_GBps": round(swap_bytes / (sms / nrep * 1e-3) / 1e9, 1) if sms else None,
+                       "lu_roofline": {"bound": "mfma", "achieved": round(upd_tf, 3),
+                                       "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                                       "frac": round(upd_tf / PEAK_FP64_MFMA, 4),
+                                       "kernel": "dgemm_mfma_kernel<false,true> (LU A22 update)",
+                                       "launches": gn // nrep}})
+        log(f"LU: update {upd_tf:.2f} TFLOP/s, panels {pms / nrep:.1f} ms, "
+            f"interchanges {sms / nrep:.1f} ms ({swap_bytes / 1e9:.2f} GB)")
         del L0, LA, piv
 
     if not args.no_mixed:
@@ -608,7 +681,7 @@ def main():
         extras["spdinv"] = bench_spdinv(fla, args.inv_n)
         extras["lu_nopiv"] = bench_lu_nopiv(fla, np, args.inv_n)
 
-    cpu = None if args.no_cpu else cpu_baseline(fla, args.cpu_n)
+    cpu = None if args.no_cpu else cpu_baseline(fla, args.cpu_n, args.cpu_lu_n)
     out = {
         "metric": "Chol/LU TFLOP/s @ n=32768 FP64 vs mixed-prec+IR time-to-solve at equal bwd err",
         "value": round(tflops, 3),
@@ -634,6 +707,8 @@ def main():
         "extras": extras,
     }
     print(json.dumps(out), flush=True)
+    # release the library's streams, events and buffers while the runtime is up
+    fla.FLA_Finalize()
 
 
 if __name__ == "__main__":

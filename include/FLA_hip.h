@@ -152,7 +152,7 @@ int fla_hip_set_reserved_cus(int reserved);
    kind 0 = Cholesky trailing update (lower SYRK), 1 = LU trailing update
    (GEMM), 2 = LU panel, 3 = Cholesky panel chain, 4 = reduced-precision
    Cholesky trailing update (FP32 / BF16 MFMA), 5 = reduced-precision LU
-   trailing update. */
+   trailing update, 6 = LU row interchanges (time and launches; flops 0). */
 int fla_hip_stats_enable(int on);
 int fla_hip_stats_reset(void);
 int fla_hip_stats_get(int kind, double* flops, double* ms, long* launches);

@@ -393,6 +393,7 @@ def set_blocksizes(chol_nb=0, lu_nb=0):
 STATS_CHOL_UPDATE, STATS_LU_UPDATE, STATS_LU_PANEL, STATS_CHOL_PANEL = 0, 1, 2, 3
 STATS_MP_UPDATE = 4
 STATS_MP_LU_UPDATE = 5
+STATS_LU_SWAP = 6
 
 
 def stats_enable(on=True):

@@ -61,7 +61,8 @@ __device__ __forceinline__ void store_chunk(double* S, const double (&reg)[BM * 
 // C := alpha*op(A)*op(B) + beta*C on a BM x BN tile per workgroup of
 // WM x WN waves; each wave owns (BM/WM) x (BN/WN) = TM x TN MFMA blocks.
 template <bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void dgemm_mfma_kernel(
+__global__ __launch_bounds__(64 * WM * WN)
+__attribute__((amdgpu_waves_per_eu(WM * WN >= 8 ? 4 : 2))) void dgemm_mfma_kernel(
     int mode, int m, int n, int k, double alpha, const double* A, long lda, const double* B,
     long ldb, double beta, double* C, long ldc, int Tm, int Tn, int rpc,
     const int* __restrict__ abort_flag) {
@@ -129,18 +130,58 @@ __global__ __launch_bounds__(64 * WM * WN) void dgemm_mfma_kernel(
   // Epilogue.  v_mfma_f64_16x16x4f64 leaves D(x, y) in lane 16*(x%4) + y,
   // register x/4 (measured on gfx950); with x = column, y = row here, lane
   // holds C(i0+wr*BM/WM+a*16+li, j0+wc*BN/WN+b*16+lk+4*r) in acc[a][b][r].
+  // Interior tiles (inside C and, in lower mode, strictly below the
+  // diagonal) take a branch-free path: per 16-row block a and group of EB
+  // column blocks every C value the lane needs is loaded first and only then
+  // combined and stored, so the loads are in flight together (one memory
+  // round trip per group instead of one per element; the group size keeps
+  // the kernel within 128 VGPRs).  Edge and diagonal tiles store through
+  // per-element guards.  Addresses are the tile's base (uniform) plus a
+  // 32-bit lane offset.
+  constexpr int EB = TN >= 2 ? 2 : 1;
+  double* Ct = C + (long)i0 + (long)j0 * ldc;
+  const int ldci = (int)ldc;
+  // (32-bit lane offsets: the tile spans fewer than 2^31 elements)
+  const bool interior = (i0 + BM <= m) && (j0 + BN <= n) && (mode != 1 || i0 >= j0 + BN - 1) &&
+                        ldc < (1L << 23);
+  if (interior) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+      const int di = wr * (BM / WM) + a * 16 + li;
+#pragma unroll
+      for (int b0 = 0; b0 < TN; b0 += EB) {
+        double cv[EB][4];
+        if (beta != 0.0) {
+#pragma unroll
+          for (int b = 0; b < EB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              cv[b][r] = Ct[(unsigned)(di + (wc * (BN / WN) + (b0 + b) * 16 + lk + 4 * r) * ldci)];
+        }
+#pragma unroll
+        for (int b = 0; b < EB; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double v = alpha * acc[a][b0 + b][r];
+            Ct[(unsigned)(di + (wc * (BN / WN) + (b0 + b) * 16 + lk + 4 * r) * ldci)] =
+                (beta == 0.0) ? v : v + beta * cv[b][r];
+          }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < TM; ++a) {
-    const int i = i0 + wr * (BM / WM) + a * 16 + li;
-    if (i >= m) continue;
+    const int di = wr * (BM / WM) + a * 16 + li;
+    const int i = i0 + di;
 #pragma unroll
     for (int b = 0; b < TN; ++b) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = j0 + wc * (BN / WN) + b * 16 + lk + 4 * r;
-        if (j >= n) continue;
-        if (mode == 1 && i < j) continue;
-        double* c = C + (long)i + (long)j * ldc;
+        const int dj = wc * (BN / WN) + b * 16 + lk + 4 * r;
+        const int j = j0 + dj;
+        if (i >= m || j >= n || (mode == 1 && i < j)) continue;
+        double* c = Ct + (long)di + (long)dj * ldc;
         const double v = alpha * acc[a][b][r];
         *c = (beta == 0.0) ? v : v + beta * (*c);
       }

@@ -670,8 +670,12 @@ static int lu_panel(hipStream_t s, int M, int W, double* Ap, long lda, int k, in
 
 static int lu_gather(hipStream_t s, int ncols, double* A, long lda, const int* moved) {
   if (ncols <= 0) return FLA_HIP_OK;
+  // measurement kind 6: time of the interchange kernels (the moved bytes
+  // depend on the pivots and are counted by the caller from them)
+  void* tok = fla_stats_begin(s);
   hipLaunchKernelGGL(lu_row_gather_kernel<double>, dim3((ncols + GC - 1) / GC), dim3(256), 0, s,
                      ncols, A, lda, moved);
+  fla_stats_end(s, tok, 6, 0.0);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }

@@ -156,6 +156,9 @@ void or_dsyrk(char uplo, char trans, int n, int k, double alpha, const double* a
         return;
     }
     if (up(trans) == 'N') {
+        /* columns are independent: the OpenMP build splits them over threads
+           (each column keeps the serial loop order, so results are identical) */
+#pragma omp parallel for private(i, l) schedule(dynamic, 8) if ((double)n * n * k > 4e6)
         for (j = 0; j < n; ++j) {
             int i0 = upper ? 0 : j, i1 = upper ? j : n - 1;
             if (beta == 0.0) {
@@ -200,6 +203,7 @@ void or_dgemm(char transa, char transb, int m, int n, int k, double alpha,
     }
     if (notb) {
         if (nota) {
+#pragma omp parallel for private(i, l) schedule(dynamic, 8) if ((double)m * n * k > 4e6)
             for (j = 0; j < n; ++j) {
                 if (beta == 0.0) {
                     for (i = 0; i < m; ++i) C_(i, j) = 0.0;
@@ -214,6 +218,7 @@ void or_dgemm(char transa, char transb, int m, int n, int k, double alpha,
                 }
             }
         } else {
+#pragma omp parallel for private(i, l) schedule(dynamic, 8) if ((double)m * n * k > 4e6)
             for (j = 0; j < n; ++j)
                 for (i = 0; i < m; ++i) {
                     double temp = 0.0;
@@ -226,6 +231,7 @@ void or_dgemm(char transa, char transb, int m, int n, int k, double alpha,
         }
     } else {
         if (nota) {
+#pragma omp parallel for private(i, l) schedule(dynamic, 8) if ((double)m * n * k > 4e6)
             for (j = 0; j < n; ++j) {
                 if (beta == 0.0) {
                     for (i = 0; i < m; ++i) C_(i, j) = 0.0;
@@ -253,21 +259,50 @@ void or_dgemm(char transa, char transb, int m, int n, int k, double alpha,
     }
 }
 
-/* src/flablas/f2c/dtrsm.c: op(A)*X = alpha*B (left) or X*op(A) = alpha*B (right). */
+/* src/flablas/f2c/dtrsm.c: op(A)*X = alpha*B (left) or X*op(A) = alpha*B (right).
+   Left solves are independent per column of B, right solves per row: the
+   OpenMP build runs the unchanged loop nest on disjoint column (row) ranges,
+   so every element sees the serial sequence of operations. */
+static void dtrsm_part(int lside, int upper, int notr, int nounit, int m, int n,
+                       double alpha, const double* a, int lda, double* b, int ldb);
+
 void or_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
               double alpha, const double* a, int lda, double* b, int ldb)
 {
-    int i, j, k;
+    int i, j;
     int lside = up(side) == 'L';
     int upper = up(uplo) == 'U';
     int notr = up(transa) == 'N';
     int nounit = up(diag) == 'N';
+    int parts = 1, q;
     if (m == 0 || n == 0) return;
     if (alpha == 0.0) {
         for (j = 0; j < n; ++j)
             for (i = 0; i < m; ++i) B_(i, j) = 0.0;
         return;
     }
+#ifdef _OPENMP
+    if ((double)m * m * n > 4e6 && (lside ? n : m) >= 16) parts = 64;
+#endif
+#pragma omp parallel for schedule(dynamic, 1) if (parts > 1)
+    for (q = 0; q < parts; ++q) {
+        if (lside) {
+            int j0 = (int)((long)n * q / parts), j1 = (int)((long)n * (q + 1) / parts);
+            if (j1 > j0)
+                dtrsm_part(lside, upper, notr, nounit, m, j1 - j0, alpha, a, lda,
+                           b + (size_t)j0 * ldb, ldb);
+        } else {
+            int i0 = (int)((long)m * q / parts), i1 = (int)((long)m * (q + 1) / parts);
+            if (i1 > i0)
+                dtrsm_part(lside, upper, notr, nounit, i1 - i0, n, alpha, a, lda, b + i0, ldb);
+        }
+    }
+}
+
+static void dtrsm_part(int lside, int upper, int notr, int nounit, int m, int n,
+                       double alpha, const double* a, int lda, double* b, int ldb)
+{
+    int i, j, k;
     if (lside) {
         if (notr) {
             if (upper) {
@@ -407,6 +442,14 @@ void or_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int i
         }
     }
 }
+
+/* Threads the level-3 routines use (1 unless built with OpenMP). */
+#ifdef _OPENMP
+#include <omp.h>
+int or_threads(void) { return omp_get_max_threads(); }
+#else
+int or_threads(void) { return 1; }
+#endif
 
 /* ------------------------------------------------------------------------ */
 /* FLAME layer                                                               */

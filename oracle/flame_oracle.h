@@ -41,6 +41,12 @@ void or_dgemm(char transa, char transb, int m, int n, int k, double alpha,
               double* c, int ldc);
 void or_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx);
 
+/* Threads used by the level-3 routines: 1 for liboracle.so; the OpenMP build
+   liboracle_omp.so (same source, -fopenmp) splits independent columns / rows
+   of dgemm, dsyrk and dtrsm over threads without changing any element's
+   sequence of operations, so both builds return identical bits. */
+int or_threads(void);
+
 /* ---- FLAME layer ---- */
 /* FLA_Chol with the fla_chol_cntl2 tree: blocked var3 at nb[0], var3 at nb[1],
    ..., leaf opt_var2.  nlev = number of blocked levels (libflame: 2, {128,32}). */

@@ -361,3 +361,45 @@ def test_dpotrs_argument_checks(orc):
         L.or_dpotrs(u, n, nrhs, orc.dp(a), lda, orc.dp(b), ldb, C.byref(info))
         assert info.value == want
     assert np.array_equal(b, np.ones((4, 2)))
+
+
+def test_openmp_oracle_build_is_bit_identical(orc):
+    """oracle/liboracle_omp.so (the multi-core CPU baseline) splits the
+    level-3 routines' independent columns / rows over threads without changing
+    any element's sequence of operations: its FLA_Chol and FLA_LU_piv equal the
+    serial oracle's bit for bit, at sizes where the threaded paths run."""
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                        "liboracle_omp.so")
+    L = C.CDLL(path)
+    dp_, ip_ = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.or_chol.argtypes = [C.c_int, C.c_int, dp_, C.c_int, ip_, C.c_int]
+    L.or_lu_piv.argtypes = [C.c_int, C.c_int, dp_, C.c_int, ip_, ip_, C.c_int]
+    L.or_dtrsm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_char, C.c_int, C.c_int, C.c_double,
+                           dp_, C.c_int, dp_, C.c_int]
+    n = 700
+    for uplo in (orc.LOWER, orc.UPPER):
+        A = orc.gen_spd_c2(5, n)
+        ref, r = orc.chol(A, uplo=uplo)
+        got = np.asfortranarray(A.copy())
+        nb = np.array(orc.NB_CHOL, np.int32)
+        assert L.or_chol(uplo, n, orc.dp(got), n, orc.ip(nb), 2) == r == orc.SUCCESS
+        assert got.tobytes() == ref.tobytes()
+    A = orc.gen_lu_c2(6, n, 650)
+    ref, p, r = orc.lu_piv(A)
+    got = np.asfortranarray(A.copy())
+    q = np.zeros(650, np.int32)
+    nb = np.array(orc.NB_LU, np.int32)
+    assert L.or_lu_piv(n, 650, orc.dp(got), n, orc.ip(q), orc.ip(nb), 2) == r
+    assert got.tobytes() == ref.tobytes() and np.array_equal(p, q)
+    rng = np.random.default_rng(1)
+    T = np.asfortranarray(np.tril(rng.uniform(-1, 1, (300, 300))) + 4 * np.eye(300))
+    for side, dims in (("L", (300, 90)), ("R", (500, 300))):
+        for tr in ("N", "T"):
+            B = np.asfortranarray(rng.standard_normal(dims))
+            want = orc.dtrsm(side, "L", tr, "N", T, B, alpha=0.5)
+            got = B.copy(order="F")
+            L.or_dtrsm(side.encode(), b"L", tr.encode(), b"N", dims[0], dims[1], 0.5, orc.dp(T),
+                       300, orc.dp(got), dims[0])
+            assert got.tobytes() == want.tobytes(), (side, tr)
