@@ -153,6 +153,7 @@ typedef struct FLA_Obj_view {
 #define FLA_BUFFER_HOST 0
 #define FLA_BUFFER_DEVICE 1
 #define FLA_BUFFER_ATTACHED 2
+#define FLA_BUFFER_TREE 4 /* top-left leaf of a FLASH tree that owns its allocation */
 
 extern FLA_Obj FLA_ONE;
 extern FLA_Obj FLA_ZERO;
@@ -188,6 +189,13 @@ void* FLA_Obj_buffer_at_view(FLA_Obj obj);
 FLA_Bool FLA_Obj_is_identical(FLA_Obj A, FLA_Obj B);
 FLA_Bool FLA_Obj_has_zero_dim(FLA_Obj A);
 FLA_Bool FLA_Obj_is_device(FLA_Obj obj); /* libflame_amd: buffer in GPU memory */
+FLA_Elemtype FLA_Obj_elemtype(FLA_Obj obj);
+void* FLA_Obj_base_buffer(FLA_Obj obj);
+dim_t FLA_Obj_base_length(FLA_Obj obj);
+dim_t FLA_Obj_base_width(FLA_Obj obj);
+dim_t FLA_Obj_row_offset(FLA_Obj obj);
+dim_t FLA_Obj_col_offset(FLA_Obj obj);
+dim_t FLA_Obj_num_elem_alloc(FLA_Obj obj);
 
 /* ---- views (FLA_View.c) ---- */
 FLA_Error FLA_Part_2x2(FLA_Obj A, FLA_Obj* A11, FLA_Obj* A12, FLA_Obj* A21, FLA_Obj* A22,
@@ -242,6 +250,81 @@ FLA_Error FLA_Gemm_external(FLA_Trans transa, FLA_Trans transb, FLA_Obj alpha, F
                             FLA_Obj B, FLA_Obj beta, FLA_Obj C);
 FLA_Error FLA_Copy(FLA_Obj A, FLA_Obj B);
 FLA_Error FLA_Set(FLA_Obj alpha, FLA_Obj A);
+
+/* ---- hierarchical (FLASH) objects ----
+   src/base/flamec/hierarchy/main/FLASH_Obj.c, FLASH_View.c, FLASH_Copy_other.c.
+   Inner nodes (elemtype FLA_MATRIX) hold host arrays of child objects in
+   units of blocks; the leaves are column-major blocks of one device
+   allocation, stored in the reference's order (block-column-major at every
+   level, leading dimension = the leaf's row count). */
+#define FLASH_OBJ_PTR_AT(A) ((FLA_Obj*)FLA_Obj_buffer_at_view(A))
+FLA_Error FLASH_Obj_create(FLA_Datatype datatype, dim_t m, dim_t n, dim_t depth, dim_t* b_mn,
+                           FLA_Obj* H);
+FLA_Error FLASH_Obj_create_ext(FLA_Datatype datatype, dim_t m, dim_t n, dim_t depth, dim_t* b_m,
+                               dim_t* b_n, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_without_buffer(FLA_Datatype datatype, dim_t m, dim_t n, dim_t depth,
+                                          dim_t* b_mn, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_without_buffer_ext(FLA_Datatype datatype, dim_t m, dim_t n,
+                                              dim_t depth, dim_t* b_m, dim_t* b_n, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_conf_to(FLA_Trans trans, FLA_Obj H, FLA_Obj* H_new);
+FLA_Error FLASH_Obj_create_hier_conf_to_flat(FLA_Trans trans, FLA_Obj F, dim_t depth,
+                                             dim_t* b_mn, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_hier_conf_to_flat_ext(FLA_Trans trans, FLA_Obj F, dim_t depth,
+                                                 dim_t* b_m, dim_t* b_n, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_flat_conf_to_hier(FLA_Trans trans, FLA_Obj H, FLA_Obj* F);
+FLA_Error FLASH_Obj_create_copy_of(FLA_Trans trans, FLA_Obj H_cur, FLA_Obj* H_new);
+FLA_Error FLASH_Obj_create_hier_copy_of_flat(FLA_Obj F, dim_t depth, dim_t* b_mn, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_hier_copy_of_flat_ext(FLA_Obj F, dim_t depth, dim_t* b_m,
+                                                 dim_t* b_n, FLA_Obj* H);
+FLA_Error FLASH_Obj_create_flat_copy_of_hier(FLA_Obj H, FLA_Obj* F);
+void FLASH_Obj_free(FLA_Obj* H);
+void FLASH_Obj_free_without_buffer(FLA_Obj* H);
+void FLASH_Obj_free_hierarchy(FLA_Obj* H);
+void* FLASH_Obj_extract_buffer(FLA_Obj H);
+FLA_Error FLASH_Obj_attach_buffer(void* buffer, dim_t rs, dim_t cs, FLA_Obj* H);
+FLA_Error FLASH_Obj_attach_buffer_hierarchy(FLA_Obj F, FLA_Obj* H);
+FLA_Error FLASH_Obj_flatten(FLA_Obj H, FLA_Obj F);
+FLA_Error FLASH_Obj_hierarchify(FLA_Obj F, FLA_Obj H);
+FLA_Error FLASH_Copy_flat_to_hier(FLA_Obj F, dim_t i, dim_t j, FLA_Obj H);
+FLA_Error FLASH_Copy_hier_to_flat(dim_t i, dim_t j, FLA_Obj H, FLA_Obj F);
+FLA_Error FLASH_Copy(FLA_Obj A, FLA_Obj B);
+FLA_Datatype FLASH_Obj_datatype(FLA_Obj H);
+dim_t FLASH_Obj_depth(FLA_Obj H);
+dim_t FLASH_Obj_blocksizes(FLA_Obj H, dim_t* b_m, dim_t* b_n);
+dim_t FLASH_Obj_base_scalar_length(FLA_Obj H);
+dim_t FLASH_Obj_base_scalar_width(FLA_Obj H);
+dim_t FLASH_Obj_scalar_length(FLA_Obj H);
+dim_t FLASH_Obj_scalar_width(FLA_Obj H);
+dim_t FLASH_Obj_scalar_min_dim(FLA_Obj H);
+dim_t FLASH_Obj_scalar_max_dim(FLA_Obj H);
+dim_t FLASH_Obj_scalar_vector_dim(FLA_Obj H);
+dim_t FLASH_Obj_scalar_row_offset(FLA_Obj H);
+dim_t FLASH_Obj_scalar_col_offset(FLA_Obj H);
+dim_t FLASH_Obj_scalar_length_tl(FLA_Obj H);
+dim_t FLASH_Obj_scalar_width_tl(FLA_Obj H);
+
+/* ---- FLASH front-ends (hierarchical objects; flat objects act as one leaf) ---- */
+FLA_Error FLASH_Chol(FLA_Uplo uplo, FLA_Obj A);
+FLA_Error FLASH_Chol_solve(FLA_Uplo uplo, FLA_Obj A, FLA_Obj B, FLA_Obj X);
+FLA_Error FLASH_LU_piv(FLA_Obj A, FLA_Obj p);
+FLA_Error FLASH_LU_piv_solve(FLA_Obj A, FLA_Obj p, FLA_Obj B, FLA_Obj X);
+FLA_Error FLASH_Apply_pivots(FLA_Side side, FLA_Trans trans, FLA_Obj p, FLA_Obj A);
+FLA_Error FLASH_LU_nopiv(FLA_Obj A);
+FLA_Error FLASH_LU_nopiv_solve(FLA_Obj A, FLA_Obj B, FLA_Obj X);
+FLA_Error FLASH_LU_find_zero_on_diagonal(FLA_Obj A);
+FLA_Error FLASH_LU_incpiv(FLA_Obj A, FLA_Obj p, FLA_Obj L);
+FLA_Error FLASH_LU_incpiv_create_hier_matrices(FLA_Obj A_flat, dim_t depth, dim_t* b_flash,
+                                               dim_t b_alg, FLA_Obj* A, FLA_Obj* p, FLA_Obj* L);
+dim_t FLASH_LU_incpiv_determine_alg_blocksize(FLA_Obj A);
+FLA_Error FLASH_FS_incpiv(FLA_Obj A, FLA_Obj p, FLA_Obj L, FLA_Obj b);
+FLA_Error FLASH_LU_incpiv_solve(FLA_Obj A, FLA_Obj p, FLA_Obj L, FLA_Obj B, FLA_Obj X);
+/* hierarchical level-3 front-ends used by the FLASH solves */
+FLA_Error FLASH_Trsm(FLA_Side side, FLA_Uplo uplo, FLA_Trans trans, FLA_Diag diag, FLA_Obj alpha,
+                     FLA_Obj A, FLA_Obj B);
+FLA_Error FLASH_Gemm(FLA_Trans transa, FLA_Trans transb, FLA_Obj alpha, FLA_Obj A, FLA_Obj B,
+                     FLA_Obj beta, FLA_Obj C);
+FLA_Error FLASH_Herk(FLA_Uplo uplo, FLA_Trans trans, FLA_Obj alpha, FLA_Obj A, FLA_Obj beta,
+                     FLA_Obj C);
 
 /* ---- lapack2flame (host pointers) ---- */
 int dpotrf_(char* uplo, int* n, double* buff_A, int* ldim_A, int* info);

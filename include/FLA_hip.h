@@ -107,6 +107,31 @@ int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, const int* p, 
    skipped (FLA_Apply_pivots on an m-row object). */
 int fla_hip_dlaswp_rows(int m, int ncols, double* A, int lda, int k1, int k2, const int* p,
                         int incp);
+/* ---- hierarchical (FLASH) storage ----
+   Copies ntiles pieces between tiles and the flat column-major matrix
+   (flat, ldf): desc holds 6 host int64 per piece -- the tile's device
+   address, its leading dimension, rows, cols, and the row and column of the
+   piece in the flat matrix.  to_tiles != 0 writes the tiles, else the flat
+   matrix.  elem_bytes 8 (double) or 4 (int).  Synchronous.  Replaces the
+   host walk of FLASH_Copy_flat_to_hier / FLASH_Copy_hier_to_flat
+   (src/base/flamec/hierarchy/main/FLASH_Copy_other.c:81-138). */
+int fla_hip_tile_copy(int ntiles, const long long* desc, void* flat, long ldf, int elem_bytes,
+                      int to_tiles);
+
+/* ---- LU with incremental pivoting (FLASH_LU_incpiv) ----
+   On the flat m x n matrix A whose b x b tiles are the hierarchical object's
+   blocks (m and n multiples of b), inner block nb_alg.  p: the flattened
+   pivot object, m x (n/b) ints, tile (i,k) at rows i*b, column k; L: the
+   flattened L object, m x (n/b)*nb_alg, tile (i,k) at rows i*b, columns
+   k*nb_alg.  Synchronous.  Replaces FLASH_LU_incpiv_var2
+   (src/lapack/dec/lu/incpiv/zaux/flamec/flash/FLASH_LU_incpiv_var2.c:13-114)
+   and FLASH_FS_incpiv_aux1 (FLASH_FS_incpiv_aux1.c:13-118). */
+int fla_hip_dgetrf_incpiv(int m, int n, double* A, int lda, int b, int nb_alg, int* p, int ldp,
+                          double* L, int ldl);
+/* The forward substitution of those transformations on the m x nrhs X. */
+int fla_hip_dfs_incpiv(int m, int n, const double* A, int lda, int b, int nb_alg, const int* p,
+                       int ldp, const double* L, int ldl, int nrhs, double* X, int ldx);
+
 /* ---- LAPACK auxiliaries on device matrices ----
    norm: '1' / 'O' (max column sum), 'I' (max row sum), 'M' (max |a_ij|),
    'F' / 'E' (Frobenius); *value receives the norm (synchronizes).  NaN
@@ -127,7 +152,8 @@ int fla_hip_dlaset(int uplo, int m, int n, double alpha, double beta, double* A,
 int fla_hip_shift_pivots(int ptype, int m, int* p);
 
 /* ---- tuning ---- */
-/* Block sizes; 0 keeps current.  chol_nb: outer Cholesky block (multiple of
+/* Block sizes; 0 keeps current, -1 restores the default.  chol_nb: outer
+   Cholesky block (default: chosen from n; else a multiple of
    128, <= 1024).  lu_nb: 32..128 (multiple of 32) = one panel of that width
    per step; 256 (default) = blocks of two 128-column panels, whose trailing
    update contracts over 256. */

@@ -59,6 +59,7 @@ def lib():
                                "(libflame_amd has no CPU fallback)")
         L = C.CDLL(LIB_PATH)
         vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        ulp = C.POINTER(C.c_ulong)
         O = FLA_Obj
         OP = C.POINTER(FLA_Obj)
         sig = {
@@ -98,6 +99,58 @@ def lib():
                                   C.POINTER(C.c_float), ip, ip]),
             "dsgesv_": (C.c_int, [ip, ip, dp, ip, ip, dp, ip, dp, ip, dp,
                                   C.POINTER(C.c_float), ip, ip]),
+            "FLA_Obj_elemtype": (C.c_int, [O]),
+            "FLA_Part_2x1": (C.c_int, [O, OP, OP, C.c_ulong, C.c_int]),
+            "FLA_Part_1x2": (C.c_int, [O, OP, OP, C.c_ulong, C.c_int]),
+            "FLASH_Obj_create": (C.c_int, [C.c_int, C.c_ulong, C.c_ulong, C.c_ulong, ulp, OP]),
+            "FLASH_Obj_create_ext": (C.c_int, [C.c_int, C.c_ulong, C.c_ulong, C.c_ulong, ulp,
+                                               ulp, OP]),
+            "FLASH_Obj_create_without_buffer": (C.c_int, [C.c_int, C.c_ulong, C.c_ulong,
+                                                          C.c_ulong, ulp, OP]),
+            "FLASH_Obj_create_without_buffer_ext": (C.c_int, [C.c_int, C.c_ulong, C.c_ulong,
+                                                              C.c_ulong, ulp, ulp, OP]),
+            "FLASH_Obj_create_conf_to": (C.c_int, [C.c_int, O, OP]),
+            "FLASH_Obj_create_hier_copy_of_flat": (C.c_int, [O, C.c_ulong, ulp, OP]),
+            "FLASH_Obj_create_flat_copy_of_hier": (C.c_int, [O, OP]),
+            "FLASH_Obj_free": (None, [OP]), "FLASH_Obj_free_without_buffer": (None, [OP]),
+            "FLASH_Obj_extract_buffer": (vp, [O]),
+            "FLASH_Obj_attach_buffer": (C.c_int, [vp, C.c_ulong, C.c_ulong, OP]),
+            "FLASH_Obj_flatten": (C.c_int, [O, O]), "FLASH_Obj_hierarchify": (C.c_int, [O, O]),
+            "FLASH_Copy_flat_to_hier": (C.c_int, [O, C.c_ulong, C.c_ulong, O]),
+            "FLASH_Copy_hier_to_flat": (C.c_int, [C.c_ulong, C.c_ulong, O, O]),
+            "FLASH_Copy": (C.c_int, [O, O]),
+            "FLASH_Obj_depth": (C.c_ulong, [O]),
+            "FLASH_Obj_blocksizes": (C.c_ulong, [O, ulp, ulp]),
+            "FLASH_Obj_base_scalar_length": (C.c_ulong, [O]),
+            "FLASH_Obj_base_scalar_width": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_length": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_width": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_min_dim": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_row_offset": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_col_offset": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_length_tl": (C.c_ulong, [O]),
+            "FLASH_Obj_scalar_width_tl": (C.c_ulong, [O]),
+            "FLASH_Chol": (C.c_int, [C.c_int, O]),
+            "FLASH_Chol_solve": (C.c_int, [C.c_int, O, O, O]),
+            "FLASH_LU_piv": (C.c_int, [O, O]),
+            "FLASH_LU_piv_solve": (C.c_int, [O, O, O, O]),
+            "FLASH_Apply_pivots": (C.c_int, [C.c_int, C.c_int, O, O]),
+            "FLASH_LU_nopiv": (C.c_int, [O]),
+            "FLASH_LU_nopiv_solve": (C.c_int, [O, O, O]),
+            "FLASH_LU_find_zero_on_diagonal": (C.c_int, [O]),
+            "FLASH_LU_incpiv": (C.c_int, [O, O, O]),
+            "FLASH_LU_incpiv_create_hier_matrices": (C.c_int, [O, C.c_ulong, ulp, C.c_ulong, OP,
+                                                               OP, OP]),
+            "FLASH_LU_incpiv_determine_alg_blocksize": (C.c_ulong, [O]),
+            "FLASH_FS_incpiv": (C.c_int, [O, O, O, O]),
+            "FLASH_LU_incpiv_solve": (C.c_int, [O, O, O, O, O]),
+            "FLASH_Trsm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, O, O, O]),
+            "fla_hip_dgetrf_incpiv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                                                vp, C.c_int, vp, C.c_int]),
+            "fla_hip_dfs_incpiv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int, vp,
+                                             C.c_int, vp, C.c_int, C.c_int, vp, C.c_int]),
+            "fla_hip_tile_copy": (C.c_int, [C.c_int, C.POINTER(C.c_longlong), vp, C.c_long,
+                                            C.c_int, C.c_int]),
             "fla_hip_init": (C.c_int, []), "fla_hip_finalize": (C.c_int, []),
             "fla_hip_synchronize": (C.c_int, []),
             "fla_hip_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
@@ -197,6 +250,7 @@ class MpOpts(C.Structure):
 
 FLA_MP_FP32, FLA_MP_BF16 = 0, 1
 FLA_IR_CLASSIC, FLA_IR_GMRES = 0, 1
+FLA_HIP_EARG = 9001
 FLA_HIP_ERANGE = 9004
 
 
@@ -322,6 +376,71 @@ class Obj:
             pass
 
 
+class FlashObj:
+    """A hierarchical (storage-by-blocks) object made by FLASH_Obj_create:
+    m x n scalars in blocks of b (one size per level, or (b_m, b_n) lists),
+    leaves in one device allocation."""
+
+    def __init__(self, m, n, b, datatype=FLA_DOUBLE, b_n=None, buffer=True):
+        bm = list(b) if isinstance(b, (list, tuple)) else [int(b)]
+        bn = list(b_n) if b_n is not None else bm
+        self.depth = len(bm)
+        self.o = FLA_Obj()
+        self.buffer = buffer
+        am = (C.c_ulong * self.depth)(*bm)
+        an = (C.c_ulong * self.depth)(*bn)
+        fn = lib().FLASH_Obj_create_ext if buffer else lib().FLASH_Obj_create_without_buffer_ext
+        r = fn(datatype, int(m), int(n), self.depth, am, an, C.byref(self.o))
+        if r != FLA_SUCCESS:
+            raise RuntimeError(f"FLASH_Obj_create failed ({r})")
+        self.m, self.n, self.datatype = int(m), int(n), datatype
+
+    @classmethod
+    def wrap(cls, o, m, n, datatype=FLA_DOUBLE):
+        """Takes ownership of a hierarchical FLA_Obj made by the library."""
+        self = cls.__new__(cls)
+        self.o, self.m, self.n, self.datatype, self.buffer = o, int(m), int(n), datatype, True
+        self.depth = int(lib().FLASH_Obj_depth(o))
+        return self
+
+    @classmethod
+    def from_host(cls, a, b, b_n=None):
+        a = np.asarray(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        dt = FLA_INT if a.dtype == np.int32 else FLA_DOUBLE
+        self = cls(a.shape[0], a.shape[1], b, dt, b_n)
+        F = DeviceMatrix.from_host(a, np.int32 if dt == FLA_INT else np.float64)
+        Fo = F.obj(dt)  # the wrapper must outlive the call (it frees the base)
+        check_fla(lib().FLASH_Copy_flat_to_hier(Fo.o, 0, 0, self.o), "hierarchify")
+        return self
+
+    def download(self):
+        dtype = np.int32 if self.datatype == FLA_INT else np.float64
+        F = DeviceMatrix(self.m, self.n, dtype)
+        Fo = F.obj(self.datatype)
+        check_fla(lib().FLASH_Copy_hier_to_flat(0, 0, self.o, Fo.o), "flatten")
+        return F.download()
+
+    def free(self):
+        if self.o.base:
+            if self.buffer:
+                lib().FLASH_Obj_free(C.byref(self.o))
+            else:
+                lib().FLASH_Obj_free_without_buffer(C.byref(self.o))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def check_fla(r, what):
+    if r != FLA_SUCCESS:
+        raise RuntimeError(f"{what} failed ({r})")
+
+
 def FLA_Init():
     lib().FLA_Init()
 
@@ -387,6 +506,7 @@ def synchronize():
 
 
 def set_blocksizes(chol_nb=0, lu_nb=0):
+    """0 keeps a block size, -1 restores its default."""
     check(lib().fla_hip_set_blocksizes(chol_nb, lu_nb), "set_blocksizes")
 
 

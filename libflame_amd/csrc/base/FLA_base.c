@@ -336,13 +336,27 @@ dim_t FLA_Obj_vector_dim(FLA_Obj obj) { return obj.m == 1 ? obj.n : obj.m; }
 dim_t FLA_Obj_vector_inc(FLA_Obj obj) { return obj.m == 1 ? obj.base->cs : obj.base->rs; }
 dim_t FLA_Obj_row_stride(FLA_Obj obj) { return obj.base->rs; }
 dim_t FLA_Obj_col_stride(FLA_Obj obj) { return obj.base->cs; }
-dim_t FLA_Obj_elem_size(FLA_Obj obj) { return elem_size_of(FLA_Obj_datatype(obj)); }
+dim_t fla_datatype_size(FLA_Datatype dt) { return elem_size_of(dt); }
+
+/* Elements of a hierarchical node (elemtype FLA_MATRIX) are FLA_Obj. */
+static dim_t obj_elem_size(const FLA_Base_obj* b) {
+  return b->elemtype == FLA_MATRIX ? (dim_t)sizeof(FLA_Obj) : elem_size_of(b->datatype);
+}
+dim_t FLA_Obj_elem_size(FLA_Obj obj) { return obj.base ? obj_elem_size(obj.base) : 0; }
 
 void* FLA_Obj_buffer_at_view(FLA_Obj obj) {
   char* p = (char*)obj.base->buffer;
   if (p == NULL) return NULL;
-  return p + (obj.offn * obj.base->cs + obj.offm * obj.base->rs) * elem_size_of(obj.base->datatype);
+  return p + (obj.offn * obj.base->cs + obj.offm * obj.base->rs) * obj_elem_size(obj.base);
 }
+
+FLA_Elemtype FLA_Obj_elemtype(FLA_Obj obj) { return obj.base ? obj.base->elemtype : FLA_SCALAR; }
+void* FLA_Obj_base_buffer(FLA_Obj obj) { return obj.base ? obj.base->buffer : NULL; }
+dim_t FLA_Obj_base_length(FLA_Obj obj) { return obj.base ? obj.base->m : 0; }
+dim_t FLA_Obj_base_width(FLA_Obj obj) { return obj.base ? obj.base->n : 0; }
+dim_t FLA_Obj_row_offset(FLA_Obj obj) { return obj.offm; }
+dim_t FLA_Obj_col_offset(FLA_Obj obj) { return obj.offn; }
+dim_t FLA_Obj_num_elem_alloc(FLA_Obj obj) { return obj.base ? obj.base->n_elem_alloc : 0; }
 
 FLA_Bool FLA_Obj_is_identical(FLA_Obj A, FLA_Obj B) {
   if (A.base != B.base) return FALSE;

@@ -39,4 +39,13 @@ typedef struct {
 FLA_Error fla_devmat_acquire(FLA_Obj A, int copy_in, fla_devmat* d);
 FLA_Error fla_devmat_release(fla_devmat* d, int copy_out);
 
+/* Bytes per element of a scalar datatype. */
+dim_t fla_datatype_size(FLA_Datatype dt);
+
+/* Copies between the leaves covering the scalar region (i, j, m, n) of a
+   hierarchical view and a device column-major matrix (to_leaves: write the
+   leaves).  Asynchronous on the library's update stream. */
+FLA_Error fla_hier_copy(FLA_Obj H, dim_t i, dim_t j, dim_t m, dim_t n, void* dev, int ld,
+                        int to_leaves);
+
 #endif

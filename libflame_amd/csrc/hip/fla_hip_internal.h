@@ -40,8 +40,10 @@ struct FlaHipCtx {
   static constexpr int kEvents = 8;
   hipEvent_t ev[kEvents] = {};
   // Grow-only device workspaces: slot 0 general scratch (dtrsm, dlaswp),
-  // slot 1 the mixed-precision solver's buffers, slot 2 its solve scratch.
-  static constexpr int kSlots = 4;
+  // slot 1 the mixed-precision solver's buffers, slot 2 its solve scratch,
+  // slot 3 generators and small reductions, slot 4 the hierarchical-storage
+  // copy tables.
+  static constexpr int kSlots = 5;
   void* work[kSlots] = {};
   size_t work_bytes[kSlots] = {};
   // Small persistent device scratch: [0] info word, [1..] counters.
@@ -188,11 +190,13 @@ int launch_trsm_llnu(hipStream_t s, int ncols, int nb, const double* L11, long l
 // X := inv(op(T)) X for na x nrhs X (left side), W = inverted 128x128 diagonal
 // blocks of T in T's orientation; one persistent launch per 16 columns.
 // done/err are two device ints the call resets; *err != 0 after a bounded
-// wait expired.
+// wait expired.  With ill != nullptr, block k with ill[k] != 0 is solved by
+// substitution through op(T_kk) itself (unit: unit diagonal) instead of by
+// its inverse.
 constexpr int kNarrowRhs = 32;
 int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, int nrhs,
                             const double* T, long ldt, const double* W, double* X, long ldx,
-                            int* done, int* err);
+                            int* done, int* err, const int* ill = nullptr, bool unit = false);
 // Inverts the 128x128 diagonal blocks of a triangle (lower or upper storage)
 // into W, block k at W + k*128*128 (ld 128, A's orientation, other triangle 0).
 // Substitution (no block inverses, the reference's dtrsm_ order): A12 :=
@@ -202,8 +206,11 @@ int launch_trsm_llnu_subst(hipStream_t s, int ncols, int nb, const double* L11, 
                            double* A12, long lda);
 int launch_trsm_runn_rows(hipStream_t s, int nrows, int nb, const double* U11, long ldu,
                           double* A21, long lda);
+// With ill != nullptr, ill[k] (one int per block) is set to 1 when block k's
+// condition number exceeds the bound up to which the few-right-hand-side
+// solve multiplies by the inverse, else 0.
 int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool upper, bool unit,
-                        double* W);
+                        double* W, int* ill = nullptr);
 // FP64 factor of columns [k, k+kb) of a lower matrix (rows k..n-1): diagonal
 // factors, panel solves and in-panel updates, all on stream s.
 int chol_panel_f64(hipStream_t s, int n, double* A, long lda, int k, int kb, int* info);

@@ -249,20 +249,19 @@ extern "C" const char* fla_hip_error_string(int status) {
     case FLA_HIP_EARG: return "invalid argument";
     case FLA_HIP_ENODEV: return "no gfx950 device";
     case FLA_HIP_ESPIN: return "inter-workgroup wait expired";
+    case FLA_HIP_ERANGE: return "input does not fit the reduced precision";
     default: return hipGetErrorString((hipError_t)status);
   }
 }
 
+// 0 leaves a block size as it is; -1 restores the default (chol_nb: chosen
+// from n; lu_nb: 256).
 extern "C" int fla_hip_set_blocksizes(int chol_nb, int lu_nb) {
-  if (chol_nb < 0 || lu_nb < 0) return FLA_HIP_EARG;
-  if (chol_nb > 0) {
-    if (chol_nb % 128 != 0 || chol_nb > 1024) return FLA_HIP_EARG;
-    g_ctx.chol_nb = chol_nb;
-  }
-  if (lu_nb > 0) {
-    if (lu_nb % 32 != 0 || (lu_nb > 128 && lu_nb != 256)) return FLA_HIP_EARG;
-    g_ctx.lu_nb = lu_nb;
-  }
+  if (chol_nb < -1 || lu_nb < -1) return FLA_HIP_EARG;
+  if (chol_nb > 0 && (chol_nb % 128 != 0 || chol_nb > 1024)) return FLA_HIP_EARG;
+  if (lu_nb > 0 && (lu_nb % 32 != 0 || (lu_nb > 128 && lu_nb != 256))) return FLA_HIP_EARG;
+  if (chol_nb != 0) g_ctx.chol_nb = chol_nb > 0 ? chol_nb : 0;
+  if (lu_nb != 0) g_ctx.lu_nb = lu_nb > 0 ? lu_nb : 256;
   return FLA_HIP_OK;
 }
 

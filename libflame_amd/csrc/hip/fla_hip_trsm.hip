@@ -2,13 +2,14 @@
 //
 // General dtrsm (all side/uplo/trans/diag combinations of dtrsm_,
 // src/flablas/f2c/dtrsm.c, used by FLA_Trsm_external and the solve paths
-// FLA_Chol_solve / FLA_LU_piv_solve / dpotrs_ / dgetrs_): the diagonal
-// 128x128 blocks of the triangle are inverted in parallel (one workgroup per
-// block), then a block substitution runs as a chain of MFMA updates:
-//   X_k := op(inv(T_kk)) * (B_k - sum_{j<k} op(T)_kj X_j)
-// Each product is an MFMA GEMM whose contracted dimension is one 128-block,
-// so X_k can overwrite B_k in place (every workgroup owns whole rows/columns
-// of the block it writes).
+// FLA_Chol_solve / FLA_LU_piv_solve / dpotrs_ / dgetrs_), blocked by 128:
+//   X_k := inv(op(T_kk)) * (B_k - sum_{j<k} op(T)_kj X_j)
+// The diagonal-block solves substitute in dtrsm_'s order (block staged in
+// LDS, one lane per right-hand side); the updates are MFMA GEMMs.  With at
+// most 32 right-hand sides on the left, one persistent launch per 16 columns
+// (fla_hip_trsv.hip) does the whole solve; it multiplies by the inverted
+// diagonal blocks (inverted in parallel, one workgroup per block) where their
+// condition number is at most kInvCondMax and substitutes through the rest.
 //
 // LU panel: A12 := inv(tril_unit(L11)) * A12 (FLA_LU_piv_blk_var5.c:88-91,
 // dtrsm_('L','L','N','U')): for a full 128-column panel, a blocked
@@ -22,6 +23,10 @@ namespace {
 
 constexpr int TB = 128;       // diagonal block order
 constexpr int TLD = TB + 1;   // LDS leading dimension
+// Largest condition number of a diagonal block that the few-right-hand-side
+// solve multiplies by its inverse (the block order: the inverse's error is
+// then of the order of a substitution's); beyond it the solve substitutes.
+constexpr double kInvCondMax = 128.0;
 
 // Inverts the nbk x nbk diagonal block k of a triangular matrix (lower or
 // upper storage, unit or non-unit diagonal) into W (ld TB per block, same
@@ -32,10 +37,30 @@ constexpr int TLD = TB + 1;   // LDS leading dimension
 // with the four 32x32 diagonal blocks solved by one wave each (lane = column
 // of the inverse, forward substitution, L read by broadcast) and the
 // off-diagonal blocks at sizes 64 and 128 formed as register-held products.
+// Abs sums of the 128 x 128 lower block in S: thread t < 128 sums column t,
+// thread 128 + r sums row r (out[t]).
+__device__ __forceinline__ void block_abs_sums(const double* S, int t, double* out) {
+  double sum = 0.0;
+  if (t < TB) {
+    for (int i = 0; i < TB; ++i) sum += fabs(S[t * TLD + i]);
+  } else {
+    const int r = t - TB;
+    for (int c = 0; c < TB; ++c) sum += fabs(S[c * TLD + r]);
+  }
+  out[t] = sum;
+}
+
+// When ill != nullptr, ill[k] = 1 if the condition number of block k,
+// max(||T_kk||_1 ||inv(T_kk)||_1, ||T_kk||_inf ||inv(T_kk)||_inf), exceeds
+// kInvCondMax (or is not finite), else 0: a solve that multiplies by the
+// inverse has a backward error that grows with that number, so the solves
+// substitute through such blocks instead.
 __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* __restrict__ A,
                                                            long lda, int upper, int unit,
-                                                           double* __restrict__ W) {
+                                                           double* __restrict__ W,
+                                                           int* __restrict__ ill) {
   __shared__ double S[TB * TLD];
+  __shared__ double nrm[2][2 * TB];
   const int k = blockIdx.x;
   const int r0 = k * TB;
   const int nbk = (n - r0) < TB ? (n - r0) : TB;
@@ -61,7 +86,26 @@ __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const double* 
     }
   }
   __syncthreads();
-  fla_lds::trtri_lower_lds(S, t);
+  if (ill) block_abs_sums(S, t, nrm[0]);
+  fla_lds::trtri_lower_lds(S, t);  // synchronizes before reading S
+  if (ill) {
+    block_abs_sums(S, t, nrm[1]);
+    __syncthreads();
+    if (t == 0) {
+      double a1 = 0.0, ai = 0.0, b1 = 0.0, bi = 0.0;
+      bool finite = true;  // an overflowed inverse holds inf or NaN (fmax would drop NaN)
+      for (int q = 0; q < TB; ++q) {
+        a1 = fmax(a1, nrm[0][q]);
+        ai = fmax(ai, nrm[0][TB + q]);
+        b1 = fmax(b1, nrm[1][q]);
+        bi = fmax(bi, nrm[1][TB + q]);
+        finite = finite && isfinite(nrm[0][q]) && isfinite(nrm[0][TB + q]) &&
+                 isfinite(nrm[1][q]) && isfinite(nrm[1][TB + q]);
+      }
+      const double kappa = fmax(a1 * b1, ai * bi);
+      ill[k] = (finite && kappa <= kInvCondMax) ? 0 : 1;
+    }
+  }
   double* Wk = W + (long)k * TB * TB;
   for (int idx = t; idx < TB * TB; idx += 256) {
     const int i = idx % TB, j = idx / TB;  // W(i,j), ld TB
@@ -153,6 +197,62 @@ __global__ __launch_bounds__(256) void trsm_subst_lds_kernel(int nrhs, int nb,
     for (int i = 0; i < TB; ++i)
       if (FULL || i < nb) col[i] = x[i];
   }
+}
+
+// Diagonal-block solve of the blocked dtrsm, by substitution with the block
+// staged in LDS and one lane per right-hand side:
+//   left side:  M = op(T_kk), the right-hand sides are the columns of B_k;
+//   right side: M = op(T_kk)', the right-hand sides are the rows of B_k
+//               (x op(T) = b  <=>  op(T)' x' = b').
+// M(i, c) = flip ? T(c, i) : T(i, c).  M lower: forward in dtrsm_'s column
+// order (x_c := x_c / M_cc, then x_i -= x_c M_ic for i > c); M upper:
+// backward.  Left-side solves divide by the diagonal and right-side solves
+// multiply by its reciprocal, as dtrsm_ does (src/flablas/f2c/dtrsm.c).
+template <bool MLOWER, bool ROWS>
+__global__ __launch_bounds__(256) void trsm_diag_subst_kernel(int nrhs, int nb,
+                                                              const double* __restrict__ T,
+                                                              long ldt, int flip, int unit,
+                                                              double* __restrict__ B, long ldb) {
+  __shared__ double Ms[TB * TB];  // Ms[c * TB + i] = M(i, c) on the strict triangle, else 0
+  __shared__ double dg[TB];       // M(c, c); 1 for a unit diagonal and beyond nb
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    const int i = e % TB, c = e / TB;
+    const bool in = MLOWER ? (i > c) : (i < c);
+    double v = 0.0;
+    if (in && i < nb && c < nb) v = flip ? T[(long)c + (long)i * ldt] : T[(long)i + (long)c * ldt];
+    Ms[e] = v;
+  }
+  for (int c = threadIdx.x; c < TB; c += 256)
+    dg[c] = (unit || c >= nb) ? 1.0 : T[(long)c * (ldt + 1)];
+  __syncthreads();
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nrhs) return;
+  double x[TB];
+#pragma unroll
+  for (int c = 0; c < TB; ++c)
+    x[c] = (c < nb) ? (ROWS ? B[(long)q + (long)c * ldb] : B[(long)c + (long)q * ldb]) : 0.0;
+#pragma unroll
+  for (int cc = 0; cc < TB; ++cc) {
+    const int c = MLOWER ? cc : TB - 1 - cc;
+    if (c < nb) {
+      if (!unit) x[c] = ROWS ? x[c] * (1.0 / dg[c]) : x[c] / dg[c];
+      if (MLOWER) {
+#pragma unroll
+        for (int i = c + 1; i < TB; ++i) x[i] -= x[c] * Ms[c * TB + i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < c; ++i) x[i] -= x[c] * Ms[c * TB + i];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < TB; ++c)
+    if (c < nb) {
+      if (ROWS)
+        B[(long)q + (long)c * ldb] = x[c];
+      else
+        B[(long)c + (long)q * ldb] = x[c];
+    }
 }
 
 // A12 := inv(L11) * A12 for L11 unit lower of order 128 (ld ldl), in place:
@@ -268,11 +368,36 @@ int launch_trsm_runn_rows(hipStream_t s, int nrows, int nb, const double* U11, l
 }
 
 int launch_trtri_blocks(hipStream_t s, int na, const double* A, long lda, bool upper, bool unit,
-                        double* W) {
+                        double* W, int* ill) {
   if (na <= 0) return FLA_HIP_OK;
   const int nblk = (na + TB - 1) / TB;
   hipLaunchKernelGGL(trtri_blocks_kernel, dim3(nblk), dim3(256), 0, s, na, A, lda, upper ? 1 : 0,
-                     unit ? 1 : 0, W);
+                     unit ? 1 : 0, W, ill);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
+// Solves the nb x nb diagonal block T (ld ldt) of a triangle against nrhs
+// right-hand sides of B in place (left: B is nb x nrhs; right: nrhs x nb).
+static int launch_diag_subst(hipStream_t s, bool left, bool upper, bool trans, bool unit, int nb,
+                             const double* T, long ldt, double* B, long ldb, int nrhs) {
+  if (nb <= 0 || nrhs <= 0) return FLA_HIP_OK;
+  const int flip = (left == trans) ? 1 : 0;
+  const bool mlower = (!upper && !flip) || (upper && flip);
+  const dim3 grid((unsigned)((nrhs + 255) / 256));
+  const int u = unit ? 1 : 0;
+  if (mlower && left)
+    hipLaunchKernelGGL((trsm_diag_subst_kernel<true, false>), grid, dim3(256), 0, s, nrhs, nb, T,
+                       ldt, flip, u, B, ldb);
+  else if (mlower)
+    hipLaunchKernelGGL((trsm_diag_subst_kernel<true, true>), grid, dim3(256), 0, s, nrhs, nb, T,
+                       ldt, flip, u, B, ldb);
+  else if (left)
+    hipLaunchKernelGGL((trsm_diag_subst_kernel<false, false>), grid, dim3(256), 0, s, nrhs, nb, T,
+                       ldt, flip, u, B, ldb);
+  else
+    hipLaunchKernelGGL((trsm_diag_subst_kernel<false, true>), grid, dim3(256), 0, s, nrhs, nb, T,
+                       ldt, flip, u, B, ldb);
   FLA_HIP_CHECK(hipGetLastError());
   return FLA_HIP_OK;
 }
@@ -289,14 +414,32 @@ __global__ void dscal_matrix_kernel(int m, int n, double alpha, double* B, long 
   *p = (alpha == 0.0) ? 0.0 : alpha * (*p);
 }
 
-extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double alpha,
-                             const double* A, int lda, double* B, int ldb) {
-  FLA_TRY(fla_ensure_ready());
-  if ((side != kLeft && side != kRight) || (uplo != kLower && uplo != kUpper)) return FLA_HIP_EARG;
-  if (m < 0 || n < 0) return FLA_HIP_EARG;
-  if (m == 0 || n == 0) return FLA_HIP_OK;
+static int dtrsm_args_ok(int side, int uplo, int m, int n, int lda, int ldb) {
+  if ((side != kLeft && side != kRight) || (uplo != kLower && uplo != kUpper)) return 0;
+  if (m < 0 || n < 0) return 0;
+  if (m == 0 || n == 0) return 1;
   const int na = (side == kLeft) ? m : n;  // order of A
-  if (lda < na || ldb < m) return FLA_HIP_EARG;
+  return (lda >= na && ldb >= m) ? 1 : 0;
+}
+
+// The few-right-hand-side path (the only one that needs a workspace).
+static bool dtrsm_narrow(int side, int n, int na) {
+  return side == kLeft && n <= kNarrowRhs && na <= kUcSolveRows;
+}
+
+// Doubles of workspace dtrsm_body needs for a triangle of order na: the
+// inverted 128-blocks, then int words: the wait counter, the wait-failure
+// flag and one condition flag per block.
+static size_t dtrsm_workspace_doubles(int na) {
+  const size_t nblk = (size_t)((na + TB - 1) / TB);
+  return nblk * TB * TB + 32 + nblk;
+}
+
+// dtrsm on checked, non-empty arguments with the caller's workspace W
+// (dtrsm_workspace_doubles(na) doubles of device memory).
+static int dtrsm_body(int side, int uplo, int trans, int diag, int m, int n, double alpha,
+                      const double* A, int lda, double* B, int ldb, double* W) {
+  const int na = (side == kLeft) ? m : n;  // order of A
   FlaHipCtx& c = fla_ctx();
   hipStream_t s = c.s_update;
   if (alpha != 1.0) {
@@ -310,15 +453,17 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
   const bool upper = (uplo == kUpper);
   const bool unit = (diag == kUnit);
   const int nblk = (na + TB - 1) / TB;
-  void* w = nullptr;
-  FLA_TRY(fla_workspace((size_t)nblk * TB * TB * sizeof(double) + 256, &w));
-  double* W = static_cast<double*>(w);
-  FLA_TRY(launch_trtri_blocks(s, na, A, lda, upper, unit, W));
   // op(A) is lower when (lower, N) or (upper, T): forward substitution.
   const bool op_lower = (!upper && !tr) || (upper && tr);
-  if (side == kLeft && n <= kNarrowRhs && na <= kUcSolveRows) {
+  if (dtrsm_narrow(side, n, na)) {
+    // Few right-hand sides: one persistent launch per 16 columns that
+    // multiplies by the inverted diagonal blocks where their condition
+    // number allows and substitutes through the others.
     int* flags = reinterpret_cast<int*>(W + (long)nblk * TB * TB);
-    FLA_TRY(launch_trsm_left_narrow(s, op_lower, tr, na, n, A, lda, W, B, ldb, flags, flags + 1));
+    int* ill = flags + 2;
+    FLA_TRY(launch_trtri_blocks(s, na, A, lda, upper, unit, W, ill));
+    FLA_TRY(launch_trsm_left_narrow(s, op_lower, tr, na, n, A, lda, W, B, ldb, flags, flags + 1,
+                                    ill, unit));
     int herr = 0;
     FLA_HIP_CHECK(hipMemcpyAsync(&herr, flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     FLA_HIP_CHECK(hipStreamSynchronize(s));
@@ -331,10 +476,9 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
       const int kb = op_lower ? q : (nblk - 1 - q);
       const int r0 = blk(kb);
       const int nbk = (na - r0) < TB ? (na - r0) : TB;
-      // X_k := op(inv(T_kk)) * B_k  (in place, contracted dim = nbk)
-      const double* Wk = W + (long)kb * TB * TB;
-      FLA_TRY(launch_dgemm_mfma(s, tr, false, 0, nbk, n, nbk, 1.0, Wk, TB, B + r0, ldb, 0.0,
-                                B + r0, ldb, nullptr));
+      // X_k := inv(op(T_kk)) * B_k by substitution, in place
+      FLA_TRY(launch_diag_subst(s, true, upper, tr, unit, nbk, A + (long)r0 * (lda + 1), lda,
+                                B + r0, ldb, n));
       // B_rest -= op(A)[rest, k] * X_k
       if (op_lower) {
         const int rr = r0 + nbk, mr = na - rr;
@@ -360,11 +504,10 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
       const int kb = fwd ? q : (nblk - 1 - q);
       const int c0 = blk(kb);
       const int nbk = (na - c0) < TB ? (na - c0) : TB;
-      const double* Wk = W + (long)kb * TB * TB;
       double* Bk = B + (long)c0 * ldb;
-      // X_k := B_k * op(inv(T_kk))   (in place, contracted dim = nbk)
-      FLA_TRY(launch_dgemm_mfma(s, false, tr, 0, m, nbk, nbk, 1.0, Bk, ldb, Wk, TB, 0.0, Bk, ldb,
-                                nullptr));
+      // X_k := B_k * inv(op(T_kk)) by substitution, in place
+      FLA_TRY(launch_diag_subst(s, false, upper, tr, unit, nbk, A + (long)c0 * (lda + 1), lda, Bk,
+                                ldb, m));
       if (fwd) {
         const int cc = c0 + nbk, nr = na - cc;
         if (nr > 0) {
@@ -384,4 +527,17 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
     }
   }
   return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double alpha,
+                             const double* A, int lda, double* B, int ldb) {
+  FLA_TRY(fla_ensure_ready());
+  if (!dtrsm_args_ok(side, uplo, m, n, lda, ldb)) return FLA_HIP_EARG;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  const int na = (side == kLeft) ? m : n;
+  void* w = nullptr;
+  if (dtrsm_narrow(side, n, na))
+    FLA_TRY(fla_workspace(dtrsm_workspace_doubles(na) * sizeof(double), &w));
+  return dtrsm_body(side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb,
+                    static_cast<double*>(w));
 }

@@ -33,6 +33,7 @@
 namespace {
 
 constexpr int NB = 128;
+constexpr int MLD = NB + 1;  // row stride of the staged triangle (substitution path)
 constexpr int RHS = 16;
 constexpr int NCOMP = 8;                 // compute waves (16 rows each)
 constexpr int NTHREADS = (NCOMP + 4) * 64;  // + four loader waves
@@ -72,11 +73,16 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
                                                                     const double* __restrict__ T,
                                                                     long ldt,
                                                                     const double* __restrict__ W,
-                                                                    double* X, long ldx,
+                                                                    const int* __restrict__ ill,
+                                                                    int unit, double* X, long ldx,
                                                                     double* __restrict__ Xm,
                                                                     int* done, int* err) {
-  __shared__ double Wf[NB * NB];   // op(inv(T_II)), fragment order
+  // Inverse path: op(inv(T_II)) in fragment order (NB*NB doubles).
+  // Substitution path: M = op(T_II) row-major with ld MLD on its strict
+  // triangle (lower when FWD), zero elsewhere, and the diagonal's reciprocals.
+  __shared__ double Wf[NB * MLD];
   __shared__ double Xf[NB * RHS];  // X_J (negated) or Y_I, fragment order
+  __shared__ double rdiag[NB];
   __shared__ int bail;
   const int nb = (n + NB - 1) / NB;
   const int I = FWD ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
@@ -84,13 +90,27 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
   const bool comp = w < NCOMP;
   const int lt = t - NCOMP * 64;  // loader thread index (0..255) when !comp
+  const bool subst = ill != nullptr && ill[I] != 0;  // uniform per workgroup
 
-  // op(W_I)(row, k) -> Wf[((row/16)*32 + k%32)*64 + (k/32)*16 + row%16]
-  const double* Wi = W + (long)I * NB * NB;
-  for (int e = t; e < NB * NB; e += NTHREADS) {
-    const int p = e & (NB - 1), q = e >> 7;  // stored W(p, q)
-    const int row = TRANS ? q : p, k = TRANS ? p : q;
-    Wf[((row >> 4) * 32 + (k & 31)) * 64 + (k >> 5) * 16 + (row & 15)] = Wi[e];
+  if (!subst) {
+    // op(W_I)(row, k) -> Wf[((row/16)*32 + k%32)*64 + (k/32)*16 + row%16]
+    const double* Wi = W + (long)I * NB * NB;
+    for (int e = t; e < NB * NB; e += NTHREADS) {
+      const int p = e & (NB - 1), q = e >> 7;  // stored W(p, q)
+      const int row = TRANS ? q : p, k = TRANS ? p : q;
+      Wf[((row >> 4) * 32 + (k & 31)) * 64 + (k >> 5) * 16 + (row & 15)] = Wi[e];
+    }
+  } else {
+    // M(i, c) = op(T)(r0 + i, r0 + c) = TRANS ? T(r0 + c, r0 + i) : T(r0 + i, r0 + c)
+    for (int e = t; e < NB * NB; e += NTHREADS) {
+      const int p = e & (NB - 1), q = e >> 7;  // stored T(r0 + p, r0 + q), p contiguous
+      const int i = TRANS ? q : p, c = TRANS ? p : q;
+      const bool in = FWD ? (i > c) : (i < c);
+      Wf[i * MLD + c] =
+          (in && r0 + p < n && r0 + q < n) ? T[(long)(r0 + p) + (long)(r0 + q) * ldt] : 0.0;
+    }
+    for (int i = t; i < NB; i += NTHREADS)
+      rdiag[i] = (unit || r0 + i >= n) ? 1.0 : 1.0 / T[(long)(r0 + i) * (ldt + 1)];
   }
   if (t == 0) bail = 0;
 
@@ -157,19 +177,67 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
     if (s + 1 < ndep && !step(s + 1, fb, fa)) return;
   }
 
-  // Y_I -> Xf as the contraction operand (its row k = 16w + li is the
-  // contraction index), then X_I = op(W_I) Y_I.
-  if (comp) {
+  d4 out = d4{0.0, 0.0, 0.0, 0.0};
+  if (!subst) {
+    // Y_I -> Xf as the contraction operand (its row k = 16w + li is the
+    // contraction index), then X_I = op(W_I) Y_I.
+    if (comp) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) Xf[(16 * (w & 1) + li) * 64 + (w >> 1) * 16 + lk + 4 * r] = acc[r];
+      for (int r = 0; r < 4; ++r)
+        Xf[(16 * (w & 1) + li) * 64 + (w >> 1) * 16 + lk + 4 * r] = acc[r];
+    }
+    __syncthreads();
+    if (comp) {
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks)
+        out = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane],
+                                                   Wf[(w * 32 + ks) * 64 + lane], out, 0, 0, 0);
+    }
+  } else {
+    // Blocked substitution over the eight 16-row sub-blocks (wave s owns
+    // sub-block s), in processing order: wave s substitutes through its
+    // 16 x 16 diagonal block in registers (x_j broadcast from lane j of each
+    // 16-lane group, dtrsm_'s column order), publishes X_s, and every later
+    // wave folds -M_ws X_s into its rows on the MFMA.
+    // The staged triangle and diagonal were written by every thread at
+    // launch; a block without dependencies reaches this point with no
+    // barrier in between.
+    __syncthreads();
+    for (int q = 0; q < NCOMP; ++q) {
+      const int sb = FWD ? q : NCOMP - 1 - q;
+      if (comp && w == sb) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+          const int j = FWD ? jj : 15 - jj;
+          const double rd = rdiag[16 * sb + j];
+          const double m = Wf[(16 * sb + li) * MLD + 16 * sb + j];
+          const bool below = FWD ? (li > j) : (li < j);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double xj = __shfl(acc[r], j + 16 * lk, 64) * rd;
+            if (li == j)
+              acc[r] = xj;
+            else if (below)
+              acc[r] -= m * xj;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xf[li * 16 + lk + 4 * r] = -acc[r];
+      }
+      __syncthreads();
+      if (comp && (FWD ? w > sb : w < sb)) {
+        // acc(col, row) += sum_k (-X_s)(k, col) M(16w + row, 16 sb + k)
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              Xf[(4 * q2 + lk) * 16 + li], Wf[(16 * w + li) * MLD + 16 * sb + 4 * q2 + lk], acc,
+              0, 0, 0);
+      }
+      __syncthreads();
+    }
+    out = acc;
   }
-  __syncthreads();
   if (comp) {
-    d4 out = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ks = 0; ks < 32; ++ks)
-      out = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], Wf[(w * 32 + ks) * 64 + lane],
-                                                 out, 0, 0, 0);
     if (row < n) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -193,9 +261,10 @@ __global__ void reset2_kernel(int* a, int* b) {
 
 template <bool FWD, bool TRANS>
 void launch_one(hipStream_t s, int nb, int na, int nr, const double* T, long ldt,
-                const double* W, double* X, long ldx, double* Xm, int* done, int* err) {
+                const double* W, const int* ill, int unit, double* X, long ldx, double* Xm,
+                int* done, int* err) {
   hipLaunchKernelGGL((trsm_left_narrow_kernel<FWD, TRANS>), dim3(nb), dim3(NTHREADS), 0, s, na, nr,
-                     T, ldt, W, X, ldx, Xm, done, err);
+                     T, ldt, W, ill, unit, X, ldx, Xm, done, err);
 }
 
 }  // namespace
@@ -204,7 +273,7 @@ void launch_one(hipStream_t s, int nb, int na, int nr, const double* T, long ldt
 // diagonal blocks (stored orientation, ld 128).  op(T) lower -> forward.
 int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, int nrhs,
                             const double* T, long ldt, const double* W, double* X, long ldx,
-                            int* done, int* err) {
+                            int* done, int* err, const int* ill, bool unit) {
   const int nb = (na + NB - 1) / NB;
   if (na > kUcSolveRows) {
     std::fprintf(stderr, "libflame_amd: the narrow triangular solve supports n <= %ld\n",
@@ -213,18 +282,19 @@ int launch_trsm_left_narrow(hipStream_t s, bool op_lower, bool trans, int na, in
   }
   // the uncached block copy; launches on one stream run one after another
   double* Xm = reinterpret_cast<double*>(fla_ctx().uc + kUcLuBytes);
+  const int u = unit ? 1 : 0;
   for (int c = 0; c < nrhs; c += RHS) {
     const int nr = (nrhs - c) < RHS ? (nrhs - c) : RHS;
     double* Xc = X + (long)c * ldx;
     hipLaunchKernelGGL(reset2_kernel, dim3(1), dim3(1), 0, s, done, err);
     if (op_lower && !trans)
-      launch_one<true, false>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
+      launch_one<true, false>(s, nb, na, nr, T, ldt, W, ill, u, Xc, ldx, Xm, done, err);
     else if (op_lower)
-      launch_one<true, true>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
+      launch_one<true, true>(s, nb, na, nr, T, ldt, W, ill, u, Xc, ldx, Xm, done, err);
     else if (!trans)
-      launch_one<false, false>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
+      launch_one<false, false>(s, nb, na, nr, T, ldt, W, ill, u, Xc, ldx, Xm, done, err);
     else
-      launch_one<false, true>(s, nb, na, nr, T, ldt, W, Xc, ldx, Xm, done, err);
+      launch_one<false, true>(s, nb, na, nr, T, ldt, W, ill, u, Xc, ldx, Xm, done, err);
     FLA_HIP_CHECK(hipGetLastError());
   }
   return FLA_HIP_OK;

@@ -43,13 +43,14 @@ __global__ void add_diag_kernel(int n, double shift, double* A, long lda) {
   if (i < n) A[(long)i + (long)i * lda] += shift;
 }
 
-__global__ void scale_rows_kernel(int m, int n, double* A, long lda) {
+// A(i, j) *= s[i]
+__global__ void scale_rows_kernel(int m, int n, const double* __restrict__ s, double* A,
+                                  long lda) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long total = (long)m * n;
   if (idx >= total) return;
   const int i = (int)(idx % m), j = (int)(idx / m);
-  const double s = pow(10.0, -(double)i / (double)(m > n ? m : n));
-  A[(long)i + (long)j * lda] *= s;
+  A[(long)i + (long)j * lda] *= s[i];
 }
 
 __global__ void shift_pivots_kernel(int to_lapack, int m, int* p) {
@@ -338,13 +339,24 @@ extern "C" int fla_hip_gram_lower(int n, int k, const double* B, int ldb, double
   return FLA_HIP_OK;
 }
 
+// The row scales 10^(-i/max(m,n)) come from the host C library's pow, so the
+// generated matrix has the same bytes as a host generator's (the device pow
+// rounds differently in the last bit for some rows).
 extern "C" int fla_hip_scale_rows_logspace(int m, int n, double* A, int lda) {
   FLA_TRY(fla_ensure_ready());
   if (m <= 0 || n <= 0) return FLA_HIP_OK;
+  std::vector<double> h((size_t)m);
+  const double mx = (double)(m > n ? m : n);
+  for (int i = 0; i < m; ++i) h[(size_t)i] = std::pow(10.0, -(double)i / mx);
+  void* w = nullptr;
+  FLA_TRY(fla_workspace_slot(3, (size_t)m * sizeof(double), &w));
+  hipStream_t s = fla_ctx().s_update;
+  FLA_HIP_CHECK(hipMemcpyAsync(w, h.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
   const long total = (long)m * n;
-  hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     fla_ctx().s_update, m, n, A, (long)lda);
+  hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m,
+                     n, static_cast<const double*>(w), A, (long)lda);
   FLA_HIP_CHECK(hipGetLastError());
+  FLA_HIP_CHECK(hipStreamSynchronize(s));  // h is released on return
   return FLA_HIP_OK;
 }
 

@@ -35,6 +35,28 @@ static FLA_Error gpu_status(int st, const char* what) {
 FLA_Error fla_devmat_acquire(FLA_Obj A, int copy_in, fla_devmat* d) {
   memset(d, 0, sizeof(*d));
   d->obj = A;
+  if (FLA_Obj_elemtype(A) == FLA_MATRIX) {
+    /* Hierarchical view: a flat column-major device copy of its scalar
+       extent, gathered from the leaves in one tile-copy launch and scattered
+       back on release. */
+    int st;
+    FLA_Error e;
+    d->m = (int)FLASH_Obj_scalar_length(A);
+    d->n = (int)FLASH_Obj_scalar_width(A);
+    d->elem = (int)fla_datatype_size(FLA_Obj_datatype(A));
+    if (d->m == 0 || d->n == 0) return FLA_SUCCESS;
+    st = fla_hip_malloc(&d->dev, (size_t)d->m * (size_t)d->n * (size_t)d->elem);
+    if (st != FLA_HIP_OK) return gpu_status(st, "flat copy of a hierarchical object");
+    d->ld = d->m;
+    d->staged = 2;
+    if (copy_in && (e = fla_hier_copy(A, 0, 0, d->m, d->n, d->dev, d->ld, 0)) != FLA_SUCCESS) {
+      fla_hip_free(d->dev);
+      d->dev = NULL;
+      d->staged = 0;
+      return e;
+    }
+    return FLA_SUCCESS;
+  }
   d->m = (int)A.m;
   d->n = (int)A.n;
   d->elem = (int)FLA_Obj_elem_size(A);
@@ -85,6 +107,18 @@ FLA_Error fla_devmat_acquire(FLA_Obj A, int copy_in, fla_devmat* d) {
 FLA_Error fla_devmat_release(fla_devmat* d, int copy_out) {
   FLA_Error r = FLA_SUCCESS;
   if (!d->staged) return FLA_SUCCESS;
+  if (d->staged == 2) {
+    /* every call that wrote the flat copy finished on the library's stream
+       before the (synchronous) scatter reads it */
+    if (copy_out && d->m > 0 && d->n > 0) {
+      if (fla_hip_synchronize() != FLA_HIP_OK) r = FLA_GPU_ERROR;
+      if (r == FLA_SUCCESS) r = fla_hier_copy(d->obj, 0, 0, d->m, d->n, d->dev, d->ld, 1);
+    }
+    fla_hip_free(d->dev);
+    d->dev = NULL;
+    d->staged = 0;
+    return r;
+  }
   if (copy_out && d->m > 0 && d->n > 0) {
     FLA_Obj A = d->obj;
     char* dst = (char*)FLA_Obj_buffer_at_view(A);
@@ -296,8 +330,9 @@ FLA_Error FLA_LU_piv(FLA_Obj A, FLA_Obj p) {
   {
     /* only min(m,n) pivots are produced: copy back that many */
     int pm = dp.m;
-    if (dp.staged) dp.m = (int)FLA_Obj_min_dim(A) < pm ? (int)FLA_Obj_min_dim(A) : pm;
-    if (dp.staged && dp.obj.m > (dim_t)dp.m) dp.obj.m = (dim_t)dp.m;
+    const int k = (int)FLASH_Obj_scalar_min_dim(A);
+    if (dp.staged) dp.m = k < pm ? k : pm;
+    if (dp.staged == 1 && dp.obj.m > (dim_t)dp.m) dp.obj.m = (dim_t)dp.m;
     fla_devmat_release(&dp, st == FLA_HIP_OK);
   }
   if (st != FLA_HIP_OK) return gpu_status(st, "FLA_LU_piv");
@@ -316,7 +351,7 @@ FLA_Error FLA_Apply_pivots(FLA_Side side, FLA_Trans trans, FLA_Obj p, FLA_Obj A)
     fprintf(stderr, "libflame_amd: FLA_Apply_pivots supports FLA_LEFT, FLA_NO_TRANSPOSE\n");
     return FLA_FAILURE;
   }
-  if (FLA_Obj_has_zero_dim(A) || FLA_Obj_vector_dim(p) == 0) return FLA_SUCCESS;
+  if (FLA_Obj_has_zero_dim(A) || FLASH_Obj_scalar_vector_dim(p) == 0) return FLA_SUCCESS;
   if ((e = require_double(A)) != FLA_SUCCESS) return e;
   if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
   if ((e = fla_devmat_acquire(p, 1, &dp)) != FLA_SUCCESS) {
@@ -324,7 +359,7 @@ FLA_Error FLA_Apply_pivots(FLA_Side side, FLA_Trans trans, FLA_Obj p, FLA_Obj A)
     return e;
   }
   st = fla_hip_dlaswp_rows(dA.m, dA.n, (double*)dA.dev, dA.ld, 0,
-                           (int)FLA_Obj_vector_dim(p) - 1, (const int*)dp.dev, 1);
+                           (int)FLASH_Obj_scalar_vector_dim(p) - 1, (const int*)dp.dev, 1);
   if (st == FLA_HIP_OK) st = fla_hip_synchronize();
   e = fla_devmat_release(&dA, st == FLA_HIP_OK);
   fla_devmat_release(&dp, 0);
@@ -465,7 +500,10 @@ FLA_Error FLA_Copy(FLA_Obj A, FLA_Obj B) {
   FLA_Error e;
   int st;
   if (FLA_Obj_has_zero_dim(A)) return FLA_SUCCESS;
-  if (A.m != B.m || A.n != B.n || FLA_Obj_elem_size(A) != FLA_Obj_elem_size(B))
+  /* scalar extents, so that flat and hierarchical objects mix */
+  if (FLASH_Obj_scalar_length(A) != FLASH_Obj_scalar_length(B) ||
+      FLASH_Obj_scalar_width(A) != FLASH_Obj_scalar_width(B) ||
+      fla_datatype_size(FLA_Obj_datatype(A)) != fla_datatype_size(FLA_Obj_datatype(B)))
     return FLA_Check_error_code(FLA_NONCONFORMAL_DIMENSIONS);
   if ((e = fla_devmat_acquire(A, 1, &dA)) != FLA_SUCCESS) return e;
   if ((e = fla_devmat_acquire(B, 0, &dB)) != FLA_SUCCESS) {
@@ -596,7 +634,7 @@ FLA_Error FLA_LU_piv_solve(FLA_Obj A, FLA_Obj p, FLA_Obj B, FLA_Obj X) {
     return e;
   }
   st = fla_hip_dlaswp_rows(dX.m, dX.n, (double*)dX.dev, dX.ld, 0,
-                           (int)FLA_Obj_vector_dim(p) - 1, (const int*)dp.dev, 1);
+                           (int)FLASH_Obj_scalar_vector_dim(p) - 1, (const int*)dp.dev, 1);
   if (st == FLA_HIP_OK)
     st = fla_hip_dtrsm(FLA_LEFT, FLA_LOWER_TRIANGULAR, FLA_NO_TRANSPOSE, FLA_UNIT_DIAG, dX.m,
                        dX.n, 1.0, (const double*)dA.dev, dA.ld, (double*)dX.dev, dX.ld);

@@ -791,6 +791,171 @@ int or_lu_piv(int m, int n, double* a, int lda, int* p, const int* nb, int nlev)
     return lu_rec(m, n, a, lda, p, nb, nlev);
 }
 
+/* ------------------------------------------------------------------------ */
+/* LU with incremental pivoting over b x b tiles: FLASH_LU_incpiv            */
+/* (src/lapack/dec/lu/incpiv/front/flamec/FLASH_LU_incpiv.c:13-39 ->         */
+/* FLASH_LU_incpiv_opt1.c:13-42 -> zaux/flamec/flash/FLASH_LU_incpiv_var2.c, */
+/* FLASH_Trsm_piv.c, FLASH_SA_LU.c, FLASH_SA_FS.c; leaves                    */
+/* zaux/flamec/fla/FLA_SA_LU_blk.c, FLA_SA_LU_unb.c, FLA_SA_FS_blk.c,        */
+/* FLA_SA_Apply_pivots.c; the diagonal tile's LU is FLA_LU_piv_task with     */
+/* fla_lu_piv_cntl_leaf = the unblocked Crout opt_var4,                       */
+/* FLA_LU_piv_copy_task.c / FLA_LU_piv_task.c).  The operations on a tile    */
+/* row (j > k) act column by column, so they run on the whole row at once.   */
+/* ------------------------------------------------------------------------ */
+
+/* FLA_SA_Apply_pivots.c:13-105: row i of C (mC rows) <-> row p[i] - (mC - i)
+   of E, for every i with p[i] != 0, in order. */
+static void sa_apply_pivots(int mC, int nC, double* c, int ldc, double* e, int lde, int mp,
+                            const int* p)
+{
+    int i, j;
+    for (i = 0; i < mp; ++i) {
+        if (p[i] != 0) {
+            int r = p[i] - (mC - i);
+            for (j = 0; j < nC; ++j) {
+                double t = C_(i, j);
+                C_(i, j) = e[(size_t)r + (size_t)j * (size_t)lde];
+                e[(size_t)r + (size_t)j * (size_t)lde] = t;
+            }
+        }
+    }
+}
+
+/* FLA_SA_LU_unb.c (double branch): LU of [U; D] with U (mU x mU) upper
+   triangular; L := triu(U) collects the rows that move, p[j] = ipiv + mU - j
+   when row ipiv of D replaces row j (|L(j,j)| < |D(ipiv,j)|), else 0. */
+static void sa_lu_unb(int mU, double* u, int ldu, int mD, double* d, int ldd, int* p, double* l,
+                      int ldl)
+{
+    int i, j, c;
+    for (c = 0; c < mU; ++c)
+        for (i = 0; i < mU; ++i) l[(size_t)i + (size_t)c * ldl] = (i <= c) ? u[(size_t)i + (size_t)c * ldu] : 0.0;
+    for (j = 0; j < mU; ++j) {
+        int ipiv = or_idamax(mD, d + (size_t)j * ldd, 1) - 1;
+        double ltmp = l[(size_t)j + (size_t)j * ldl];
+        double dtmp = d[(size_t)ipiv + (size_t)j * ldd];
+        if (fabs(ltmp) < fabs(dtmp)) {
+            for (c = 0; c < mU; ++c) {
+                double t = l[(size_t)j + (size_t)c * ldl];
+                l[(size_t)j + (size_t)c * ldl] = d[(size_t)ipiv + (size_t)c * ldd];
+                d[(size_t)ipiv + (size_t)c * ldd] = t;
+            }
+            p[j] = ipiv + mU - j;
+        } else {
+            p[j] = 0;
+        }
+        or_dscal(mD, 1.0 / l[(size_t)j + (size_t)j * ldl], d + (size_t)j * ldd, 1);
+        /* dger (f2c order): D(:, c) += D(:, j) * (-L(j, c)), c > j */
+        for (c = j + 1; c < mU; ++c) {
+            double y = l[(size_t)j + (size_t)c * ldl];
+            if (y != 0.0) {
+                double temp = -1.0 * y;
+                for (i = 0; i < mD; ++i)
+                    d[(size_t)i + (size_t)c * ldd] += d[(size_t)i + (size_t)j * ldd] * temp;
+            }
+        }
+        for (c = j; c < mU; ++c) u[(size_t)j + (size_t)c * ldu] = l[(size_t)j + (size_t)c * ldl];
+    }
+}
+
+/* FLA_SA_LU_blk.c:13-109: blocks of nb_alg columns of [U; D] (U nU x nU, D
+   mD x nU); L1_sqr is the bs x bs top-left part of L's rows s..s+bs. */
+static void sa_lu_blk(int nU, double* u, int ldu, int mD, double* d, int ldd, int* p, double* l,
+                      int ldl, int nb_alg)
+{
+    int s, bs;
+    for (s = 0; s < nU; s += bs) {
+        int nrest;
+        bs = imin(nU - s, nb_alg);
+        nrest = nU - s - bs;
+        sa_lu_unb(bs, u + (size_t)s + (size_t)s * ldu, ldu, mD, d + (size_t)s * ldd, ldd, p + s,
+                  l + s, ldl);
+        sa_apply_pivots(bs, nrest, u + (size_t)s + (size_t)(s + bs) * ldu, ldu,
+                        d + (size_t)(s + bs) * ldd, ldd, bs, p + s);
+        if (nrest > 0) {
+            or_dtrsm('L', 'L', 'N', 'U', bs, nrest, 1.0, l + s, ldl,
+                     u + (size_t)s + (size_t)(s + bs) * ldu, ldu);
+            or_dgemm('N', 'N', mD, nrest, bs, -1.0, d + (size_t)s * ldd, ldd,
+                     u + (size_t)s + (size_t)(s + bs) * ldu, ldu, 1.0,
+                     d + (size_t)(s + bs) * ldd, ldd);
+        }
+    }
+}
+
+/* FLA_SA_FS_blk.c:13-105: apply the transformations of one SA_LU (L tile
+   mL x nb_alg, D mD x mL, pivots p) to C (mL x nC) over E (mD x nC). */
+static void sa_fs_blk(int mL, const double* l, int ldl, int mD, const double* d, int ldd,
+                      const int* p, int nC, double* c, int ldc, double* e, int lde, int nb_alg)
+{
+    int s, bs;
+    for (s = 0; s < mL; s += bs) {
+        bs = imin(mL - s, nb_alg);
+        sa_apply_pivots(bs, nC, c + s, ldc, e, lde, bs, p + s);
+        or_dtrsm('L', 'L', 'N', 'U', bs, nC, 1.0, l + s, ldl, c + s, ldc);
+        or_dgemm('N', 'N', mD, nC, bs, -1.0, d + (size_t)s * ldd, ldd, c + s, ldc, 1.0, e, lde);
+    }
+}
+
+/* FLASH_LU_incpiv_var2.c:13-114: for each diagonal tile k, its LU (Crout,
+   pivots within the tile), the pivoted unit-lower solve of the tile row
+   (FLASH_Trsm_piv.c), then for each tile i below, SA_LU of [A_kk; A_ik] and
+   SA_FS of the tile rows k and i (FLASH_SA_LU.c, FLASH_SA_FS.c). */
+int or_lu_incpiv(int m, int n, double* a, int lda, int b, int nb_alg, int* p, int ldp,
+                 double* l, int ldl)
+{
+    int mt, nt, kt, k, i;
+    if (b <= 0 || nb_alg <= 0 || m % b != 0 || n % b != 0) return -2;
+    mt = m / b;
+    nt = n / b;
+    kt = imin(mt, nt);
+    for (k = 0; k < kt; ++k) {
+        double* akk = &A_(k * b, k * b);
+        int* pkk = p + (size_t)k * b + (size_t)k * ldp;
+        int nr = n - (k + 1) * b;
+        (void)lu_opd_var4(b, b, akk, lda, pkk);
+        if (nr > 0) {
+            or_apply_pivots_ln(nr, &A_(k * b, (k + 1) * b), lda, 0, b - 1, pkk, 1);
+            or_dtrsm('L', 'L', 'N', 'U', b, nr, 1.0, akk, lda, &A_(k * b, (k + 1) * b), lda);
+        }
+        for (i = k + 1; i < mt; ++i) {
+            double* aik = &A_(i * b, k * b);
+            int* pik = p + (size_t)i * b + (size_t)k * ldp;
+            double* lik = l + (size_t)i * b + (size_t)k * nb_alg * ldl;
+            sa_lu_blk(b, akk, lda, b, aik, lda, pik, lik, ldl, nb_alg);
+            if (nr > 0)
+                sa_fs_blk(b, lik, ldl, b, aik, lda, pik, nr, &A_(k * b, (k + 1) * b), lda,
+                          &A_(i * b, (k + 1) * b), lda, nb_alg);
+        }
+    }
+    return OR_SUCCESS;
+}
+
+/* FLASH_FS_incpiv.c:13-53 -> zaux/flamec/flash/FLASH_FS_incpiv_aux1.c,
+   FLASH_FS_incpiv_aux2.c: per diagonal tile, its pivots and unit-lower solve
+   (FLA_Trsv_external; column by column for several right-hand sides), then
+   SA_FS of the tiles below on the rows of x. */
+int or_fs_incpiv(int m, int n, const double* a, int lda, int b, int nb_alg, const int* p,
+                 int ldp, const double* l, int ldl, int nrhs, double* x, int ldx)
+{
+    int mt, nt, kt, k, i;
+    if (b <= 0 || nb_alg <= 0 || m % b != 0 || n % b != 0) return -2;
+    mt = m / b;
+    nt = n / b;
+    kt = imin(mt, nt);
+    for (k = 0; k < kt; ++k) {
+        const double* akk = &A_(k * b, k * b);
+        const int* pkk = p + (size_t)k * b + (size_t)k * ldp;
+        double* xk = x + (size_t)k * b;
+        or_apply_pivots_ln(nrhs, xk, ldx, 0, b - 1, pkk, 1);
+        or_dtrsm('L', 'L', 'N', 'U', b, nrhs, 1.0, akk, lda, xk, ldx);
+        for (i = k + 1; i < mt; ++i)
+            sa_fs_blk(b, l + (size_t)i * b + (size_t)k * nb_alg * ldl, ldl, b,
+                      &A_(i * b, k * b), lda, p + (size_t)i * b + (size_t)k * ldp, nrhs, xk,
+                      ldx, x + (size_t)i * b, ldx, nb_alg);
+    }
+    return OR_SUCCESS;
+}
+
 /* FLA_Shift_pivots_to (src/base/flamec/util/lapack/FLA_Shift_pivots_to.c:13-42). */
 void or_shift_pivots_to(int ptype, int m, int* p)
 {

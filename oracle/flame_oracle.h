@@ -92,6 +92,20 @@ void or_trinv(int uplo, int diag, int n, double* a, int lda);
 void or_ttmm(int uplo, int n, double* a, int lda);
 int or_spdinv(int uplo, int n, double* a, int lda, const int* nb, int nlev);
 
+/* ---- LU with incremental pivoting over b x b tiles (FLASH_LU_incpiv) ----
+   a: the flat m x n matrix, m and n multiples of b; p: the flattened
+   hierarchical pivot object, (m/b)*b x (n/b) ints, tile (i,k) at rows i*b,
+   column k; l: the flattened L object, (m/b)*b x (n/b)*nb_alg, tile (i,k) at
+   rows i*b, columns k*nb_alg.  Tiles above the diagonal of p and tiles on or
+   above the diagonal of l are not written.  Returns OR_SUCCESS, or -2 when m
+   or n is not a multiple of b. */
+int or_lu_incpiv(int m, int n, double* a, int lda, int b, int nb_alg, int* p, int ldp,
+                 double* l, int ldl);
+/* FLASH_FS_incpiv on nrhs columns of x (m rows): the incremental pivots and
+   unit-lower transformations of or_lu_incpiv, in the factorization's order. */
+int or_fs_incpiv(int m, int n, const double* a, int lda, int b, int nb_alg, const int* p,
+                 int ldp, const double* l, int ldl, int nrhs, double* x, int ldx);
+
 /* ---- error metrics ---- */
 /* ||A - L*L^T||_1 / (n * eps * ||A||_1) using only the uplo triangle of A0. */
 double or_chol_residual(int uplo, int n, const double* a0, int lda0,

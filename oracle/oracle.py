@@ -266,3 +266,50 @@ def lu_residual(A0, F, p):
     m, n = A0.shape
     p = np.ascontiguousarray(p, dtype=np.int32)
     return lib().or_lu_residual(m, n, dp(A0), max(1, m), dp(F), max(1, m), ip(p))
+
+
+def _incpiv_lib():
+    L = lib()
+    if not getattr(L, "_incpiv_sigs", False):
+        L.or_lu_incpiv.restype = C.c_int
+        L.or_lu_incpiv.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _ip,
+                                   C.c_int, _dp, C.c_int]
+        L.or_fs_incpiv.restype = C.c_int
+        L.or_fs_incpiv.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _ip,
+                                   C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_int]
+        L._incpiv_sigs = True
+    return L
+
+
+def lu_incpiv(A, b, nb_alg):
+    """FLASH_LU_incpiv on tiles of b (inner block nb_alg) of the flat A.
+    Returns (A_out, p, L, r): p is the flattened pivot object (m x n/b ints,
+    tile (i,k) at rows i*b, column k) and L the flattened L object
+    (m x (n/b)*nb_alg); their tiles that the factorization does not write
+    stay zero."""
+    A = fortran(A).copy(order="F")
+    m, n = A.shape
+    nt = max(1, n // b)
+    p = np.zeros((m, nt), dtype=np.int32, order="F")
+    Lm = np.zeros((m, nt * nb_alg), order="F")
+    r = _incpiv_lib().or_lu_incpiv(m, n, dp(A), max(1, m), b, nb_alg, ip(p), max(1, m), dp(Lm),
+                                   max(1, m))
+    return A, p, Lm, r
+
+
+def fs_incpiv(F, p, Lm, b, nb_alg, X):
+    """FLASH_FS_incpiv on the columns of X (returns a new array)."""
+    F = fortran(F)
+    X = fortran(X).copy(order="F")
+    p = np.asfortranarray(p, dtype=np.int32)
+    Lm = fortran(Lm)
+    m, n = F.shape
+    _incpiv_lib().or_fs_incpiv(m, n, dp(F), max(1, m), b, nb_alg, ip(p), max(1, m), dp(Lm),
+                               max(1, m), X.shape[1], dp(X), max(1, X.shape[0]))
+    return X
+
+
+def lu_incpiv_solve(F, p, Lm, b, nb_alg, B):
+    """FLASH_LU_incpiv_solve: forward substitution, then the upper solve."""
+    Y = fs_incpiv(F, p, Lm, b, nb_alg, B)
+    return dtrsm("L", "U", "N", "N", F, Y)

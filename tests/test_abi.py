@@ -2,6 +2,7 @@
 exports every function that include/*.h declares, the constants keep the
 reference's numeric values, and parameter errors abort as FLA_Check_error_code
 does (no GPU call is made in this file)."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -144,6 +145,16 @@ def test_lu_blocksize_values():
     for bad in (16, 160, 192, 224, 288, 512):
         assert L.fla_hip_set_blocksizes(0, bad) != 0
     assert L.fla_hip_set_blocksizes(0, 256) == 0
+    # -1 restores the defaults (Cholesky block chosen from n, lu_nb = 256)
+    assert L.fla_hip_set_blocksizes(512, 64) == 0
+    assert L.fla_hip_chol_blocksize(32768) == 512
+    assert L.fla_hip_set_blocksizes(-1, -1) == 0
+    assert L.fla_hip_chol_blocksize(32768) == 1024
+    assert L.fla_hip_chol_blocksize(8192) == 256
+    cb, lb = C.c_int(), C.c_int()
+    L.fla_hip_get_blocksizes(C.byref(cb), C.byref(lb))
+    assert (cb.value, lb.value) == (0, 256)
+    assert L.fla_hip_set_blocksizes(-2, 0) != 0
 
 
 def test_driver_lapack_argument_errors(capfd):

@@ -278,20 +278,21 @@ def test_chol_large_outer_block_vs_oracle(orc, nb):
         _, rko = orc.chol(E2)
         assert rk == rko == k
     finally:
-        fla.set_blocksizes(chol_nb=0)
+        fla.set_blocksizes(chol_nb=-1)
 
 
 # ---- solves with ill-conditioned diagonal blocks ------------------------------
 
 def _graded_upper(n, seed, decades):
-    """Upper triangle with O(1) off-diagonal entries and a diagonal graded over
-    `decades` orders of magnitude inside every 128-block (the U of an
-    ill-conditioned LU looks like this)."""
+    """U = D (I + E): the diagonal D graded over `decades` orders of magnitude
+    inside every 128-block and a small strictly upper E, so each diagonal
+    block's condition number is about 10^decades (far above what a solve by
+    inverted blocks tolerates) while the solution stays finite (the U of an
+    ill-conditioned LU, up to row scaling, looks like this)."""
     rng = np.random.default_rng(seed)
-    U = np.triu(rng.uniform(-1, 1, (n, n)))
-    d = 10.0 ** (-decades * ((np.arange(n) % 128) / 127.0))
-    U[np.diag_indices(n)] = d * rng.choice([-1.0, 1.0], n)
-    return np.asfortranarray(U)
+    d = 10.0 ** (-decades * ((np.arange(n) % 128) / 127.0)) * rng.choice([-1.0, 1.0], n)
+    E = np.triu(rng.uniform(-1.0, 1.0, (n, n)), 1) * (0.5 / np.sqrt(n))
+    return np.asfortranarray(d[:, None] * (np.eye(n) + E))
 
 
 @pytest.mark.parametrize("decades", [4, 8])

@@ -48,7 +48,7 @@ def test_chol_c1_residual_vs_oracle(orc, nb):
         L, R = np.tril(out), np.tril(ref)
         assert np.max(np.abs(L - R)) <= 1e-12 * np.max(np.abs(R))
     finally:
-        fla.set_blocksizes(chol_nb=256)
+        fla.set_blocksizes(chol_nb=-1)
 
 
 @pytest.mark.parametrize("n", [700, 2048])
