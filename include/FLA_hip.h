@@ -35,6 +35,8 @@ extern "C" {
 #define FLA_HIP_ENODEV 9002 /* no gfx950 device available */
 #define FLA_HIP_ESPIN 9003  /* a bounded inter-workgroup wait expired */
 #define FLA_HIP_ERANGE 9004 /* input does not fit the reduced precision */
+#define FLA_HIP_ENOINIT 9005   /* the library is not initialised (no-allocation entry points) */
+#define FLA_HIP_ESCRATCH 9006  /* problem exceeds a no-allocation entry point's scratch */
 
 /* sides / uplo / trans / diag use the FLAME constants (FLAME.h). */
 
@@ -70,6 +72,17 @@ int fla_hip_dsyrk(int uplo, int trans, int n, int k, double alpha, const double*
 /* B := alpha*inv(op(A))*B (left) or alpha*B*inv(op(A)) (right). */
 int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double alpha,
                   const double* A, int lda, double* B, int ldb);
+/* fla_hip_dtrsm without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops).  Same arguments, kernels and results as
+   fla_hip_dtrsm; the workspace (inverted 128-blocks, their condition flags
+   and two wait words) is a file-scope device array sized for a triangle of
+   order <= 8192 (8 MB per GPU, held until the process ends).  Requires FLA_Init / fla_hip_init
+   first (FLA_HIP_ENOINIT otherwise, no implicit initialization); a larger
+   triangle returns FLA_HIP_ESCRATCH.  Calls that overlap on one GPU (from
+   several host threads) share the array and race. */
+int fla_hip_dtrsm_noalloc(int side, int uplo, int trans, int diag, int m, int n, double alpha,
+                          const double* A, int lda, double* B, int ldb);
 
 /* ---- factorizations ---- */
 /* Blocked Cholesky, in place on the uplo triangle.  *info = FLA_SUCCESS (-1)

@@ -177,6 +177,8 @@ def lib():
                                         C.c_int, C.c_double, vp, C.c_int]),
             "fla_hip_dtrsm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_double, vp, C.c_int, vp, C.c_int]),
+            "fla_hip_dtrsm_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.c_int, C.c_double, vp, C.c_int, vp, C.c_int]),
             "fla_hip_dpotrf": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
             "fla_hip_dgetrf_nopiv": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, ip]),
             "fla_hip_dtrtri": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int]),
@@ -252,6 +254,8 @@ FLA_MP_FP32, FLA_MP_BF16 = 0, 1
 FLA_IR_CLASSIC, FLA_IR_GMRES = 0, 1
 FLA_HIP_EARG = 9001
 FLA_HIP_ERANGE = 9004
+FLA_HIP_ENOINIT = 9005
+FLA_HIP_ESCRATCH = 9006
 
 
 def dsposv(uplo, A, B, X, precision=FLA_MP_FP32, refinement=FLA_IR_CLASSIC, max_iter=30,

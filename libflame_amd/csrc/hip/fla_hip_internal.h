@@ -76,6 +76,9 @@ int fla_ensure_ready();
 // Returns a device workspace of at least `bytes` bytes (grow-only).
 int fla_workspace(size_t bytes, void** ptr);
 int fla_workspace_slot(int slot, size_t bytes, void** ptr);
+// Resolves the device address of the no-allocation scratch array for `device`
+// (called by fla_hip_init with that device current).
+int fla_noalloc_setup(int device);
 
 #define FLA_HIP_CHECK(expr)                                                    \
   do {                                                                         \

@@ -103,6 +103,7 @@ extern "C" int fla_hip_init(void) {
     FLA_HIP_CHECK(hipMemset(uc, 0, kUcBytes));  // no exchange word carries a valid tag
   }
   g_ctx.uc = uc;
+  FLA_TRY(fla_noalloc_setup(g_ctx.device));
   g_ctx.ready = true;
   static bool exit_hook = false;
   if (!exit_hook) {
@@ -250,6 +251,8 @@ extern "C" const char* fla_hip_error_string(int status) {
     case FLA_HIP_ENODEV: return "no gfx950 device";
     case FLA_HIP_ESPIN: return "inter-workgroup wait expired";
     case FLA_HIP_ERANGE: return "input does not fit the reduced precision";
+    case FLA_HIP_ENOINIT: return "library not initialised (call FLA_Init or fla_hip_init)";
+    case FLA_HIP_ESCRATCH: return "problem exceeds the no-allocation scratch limit";
     default: return hipGetErrorString((hipError_t)status);
   }
 }

@@ -541,3 +541,58 @@ extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int
   return dtrsm_body(side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb,
                     static_cast<double*>(w));
 }
+
+// ---- dtrsm without allocation ---------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops): fla_hip_dtrsm_noalloc runs the same kernels
+// as fla_hip_dtrsm with its workspace in `scratch`, a file-scope device array
+// sized for triangles of order <= kNoAllocMaxOrder (8 MB; every GPU the
+// library runs on holds its own copy until the process ends).  Host code
+// never passes `scratch` itself to a kernel (in host code the name is a
+// host-side stand-in): fla_noalloc_setup, called by fla_hip_init with the
+// device current, resolves the array's device address once per device, and
+// only that address is used.  Every call on one GPU shares the array: calls
+// that overlap on a GPU (from several host threads) race.  The wait words
+// are cleared by a kernel at the start of each solve.
+namespace {
+constexpr int kNoAllocMaxOrder = 8192;
+constexpr size_t kNoAllocDoubles = (size_t)(kNoAllocMaxOrder / TB) * TB * TB + 32 +
+                                   kNoAllocMaxOrder / TB;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kNoAllocDevices = 64;
+double* g_scratch_ptr[kNoAllocDevices] = {};
+}  // namespace
+
+int fla_noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return FLA_HIP_EARG;
+  if (g_scratch_ptr[device]) return FLA_HIP_OK;
+  void* p = nullptr;
+  FLA_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return FLA_HIP_EARG;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return FLA_HIP_OK;
+}
+
+extern "C" int fla_hip_dtrsm_noalloc(int side, int uplo, int trans, int diag, int m, int n,
+                                     double alpha, const double* A, int lda, double* B,
+                                     int ldb) {
+  FlaHipCtx& c = fla_ctx();
+  // no implicit initialization here: fla_hip_init allocates streams and buffers
+  if (!c.ready || c.device < 0 || c.device >= kNoAllocDevices || !g_scratch_ptr[c.device]) {
+    std::fprintf(stderr,
+                 "libflame_amd: fla_hip_dtrsm_noalloc needs FLA_Init / fla_hip_init first\n");
+    return FLA_HIP_ENOINIT;
+  }
+  if (!dtrsm_args_ok(side, uplo, m, n, lda, ldb)) return FLA_HIP_EARG;
+  if (m == 0 || n == 0) return FLA_HIP_OK;
+  const int na = (side == kLeft) ? m : n;
+  if (na > kNoAllocMaxOrder) {
+    std::fprintf(stderr,
+                 "libflame_amd: fla_hip_dtrsm_noalloc: triangle of order %d exceeds its scratch "
+                 "limit of %d\n",
+                 na, kNoAllocMaxOrder);
+    return FLA_HIP_ESCRATCH;
+  }
+  return dtrsm_body(side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb,
+                    g_scratch_ptr[c.device]);
+}
