@@ -501,6 +501,70 @@ def bench_lu_nopiv(fla, np, n):
     return out
 
 
+def bench_flash(fla, np, n, b, n_inc, b_inc, nb_alg):
+    """FLASH front-ends: FLASH_Chol on an n x n hierarchical object with
+    blocks of b (the flat copy, the factorization and the scatter back are
+    all inside the timed call) beside FLA_Chol on the same matrix, and
+    FLASH_LU_incpiv at n_inc with tiles of b_inc and inner block nb_alg
+    (flops 2 n^3/3, the LU convention, so the tile algorithm's extra work
+    is not counted), on U(-1,1) matrices made on the host."""
+    import ctypes as C
+    lib = fla.lib()
+    lo = fla.FLA_LOWER_TRIANGULAR
+    A0 = make_spd(fla, n, 91)
+    F = fla.DeviceMatrix(n, n)
+    Fo = F.obj()
+    bf = (C.c_ulong * 1)(b)
+    H = fla.FLA_Obj()
+    fla.check_fla(lib.FLASH_Obj_create(fla.FLA_DOUBLE, n, n, 1, bf, C.byref(H)), "create")
+    tf, th = [], []
+    for rep in range(3):
+        copy(fla, F, A0)
+        fla.synchronize()
+        t0 = time.perf_counter()
+        assert lib.FLA_Chol(lo, Fo.o) == fla.FLA_SUCCESS
+        fla.synchronize()
+        tf.append(time.perf_counter() - t0)
+        copy(fla, F, A0)
+        fla.check_fla(lib.FLASH_Copy_flat_to_hier(Fo.o, 0, 0, H), "hierarchify")
+        fla.synchronize()
+        t0 = time.perf_counter()
+        assert lib.FLASH_Chol(lo, H) == fla.FLA_SUCCESS
+        fla.synchronize()
+        th.append(time.perf_counter() - t0)
+    lib.FLASH_Obj_free(C.byref(H))
+    A0.buf.free()
+    del F, Fo
+    t_flat, t_hier = min(tf[1:]), min(th[1:])
+    out = {"chol_n": n, "chol_block": b, "flash_chol_ms": round(t_hier * 1e3, 2),
+           "fla_chol_ms": round(t_flat * 1e3, 2),
+           "flash_chol_tflops": round(n ** 3 / 3 / t_hier / 1e12, 3),
+           "flash_overhead_pct": round(100.0 * (t_hier - t_flat) / t_flat, 2)}
+    rng = np.random.default_rng(92)
+    Ah = np.asfortranarray(rng.uniform(-1.0, 1.0, size=(n_inc, n_inc)))
+    Af = fla.DeviceMatrix.from_host(Ah)
+    Afo = Af.obj()
+    ti = []
+    for rep in range(3):
+        HA, Hp, HL = fla.FLA_Obj(), fla.FLA_Obj(), fla.FLA_Obj()
+        bi = (C.c_ulong * 1)(b_inc)
+        fla.check_fla(lib.FLASH_LU_incpiv_create_hier_matrices(
+            Afo.o, 1, bi, nb_alg, C.byref(HA), C.byref(Hp), C.byref(HL)), "create incpiv")
+        fla.synchronize()
+        t0 = time.perf_counter()
+        assert lib.FLASH_LU_incpiv(HA, Hp, HL) == fla.FLA_SUCCESS
+        fla.synchronize()
+        ti.append(time.perf_counter() - t0)
+        for o in (HA, Hp, HL):
+            lib.FLASH_Obj_free(C.byref(o))
+    t = min(ti[1:])
+    out.update({"incpiv_n": n_inc, "incpiv_block": b_inc, "incpiv_nb_alg": nb_alg,
+                "incpiv_ms": round(t * 1e3, 2),
+                "incpiv_tflops": round(2 * n_inc ** 3 / 3 / t / 1e12, 3)})
+    log(f"flash: {out}")
+    return out
+
+
 def bench_adjoint_batched(fla, np, count, n):
     """Batched FP32 GP gradient: count problems of n points in [0,1]^2, l_i
     uniform on [0.05, 0.5], s = 1, noise 1e-3, y ~ N(0,1) (one launch each for
@@ -557,6 +621,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--adj-n", type=int, default=8192)
     ap.add_argument("--no-inv", action="store_true")
+    ap.add_argument("--no-flash", action="store_true")
     ap.add_argument("--inv-n", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -680,6 +745,8 @@ def main():
     if not args.no_inv:
         extras["spdinv"] = bench_spdinv(fla, args.inv_n)
         extras["lu_nopiv"] = bench_lu_nopiv(fla, np, args.inv_n)
+    if not args.no_flash:
+        extras["flash"] = bench_flash(fla, np, args.inv_n, 1024, 4096, 256, 64)
 
     cpu = None if args.no_cpu else cpu_baseline(fla, args.cpu_n, args.cpu_lu_n)
     out = {
