@@ -24,6 +24,8 @@
 //     and an MFMA GEMM E -= D1 C1 across the whole tile row.
 // The chain over i is sequential (each SA_LU updates A_kk's upper triangle),
 // as in the reference.
+#include <utility>
+
 #include "fla_hip_internal.h"
 
 namespace {
@@ -129,6 +131,163 @@ __global__ __launch_bounds__(kUnbThreads) void sa_lu_unb_kernel(int bs, double* 
   }
 }
 
+// The same leaf with D held in registers: thread t owns row t of D (mD <=
+// 256, bs <= BSMAX), L lives in LDS (row-major), and each column costs two
+// barriers: one after the per-wave pivot candidates are posted, one after
+// the interchange.  The operations and their order per element are those of
+// sa_lu_unb_kernel (the same idamax rule, the reciprocal scaling, dger_'s
+// separate multiply and add, skipped for L(j, c) == 0), so the results are
+// identical.
+struct SaCand {
+  int nan;     // last NaN row of the wave, -1 if none
+  int idx;     // first maximum of |D(i, j)| over the wave's rows after its last NaN, -1 if none
+  double best; // that maximum
+  double val;  // D(idx, j) (signed)
+};
+
+__device__ __forceinline__ double sa_wave_max(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+  return x;
+}
+
+// Column J of the register leaf (compile-time J, so that every index into
+// the row array is a constant and the array stays in VGPRs).
+template <int BSMAX, int J>
+__device__ __forceinline__ void sa_reg_column(double (&x)[BSMAX], int bs, int t, bool act,
+                                              double* Ls, SaCand* cand, double* __restrict__ U,
+                                              long ldu, int* __restrict__ p) {
+  constexpr int LS = BSMAX + 1;
+  if (J >= bs) return;  // uniform
+  const int lane = t & 63, wave = t >> 6;
+  // L(J, J) before the barrier: row J of L changes only in this column's
+  // interchange, which the pivot row's owner makes after the barrier.
+  const double ltmp = Ls[J * LS + J];
+  // ---- per-wave idamax candidates of column J ----
+  {
+    const double xj = x[J];
+    const bool isnan = act && xj != xj;
+    const unsigned long long nb = __ballot(isnan);
+    const int wnan = nb ? 63 - __builtin_clzll(nb) : -1;  // lane of the wave's last NaN
+    const double key = (act && !isnan && lane > wnan) ? fabs(xj) : -1.0;
+    const double m = sa_wave_max(key);
+    const unsigned long long bm = __ballot(m >= 0.0 && key == m);
+    const int l = bm ? __builtin_ctzll(bm) : 0;
+    const double v = __shfl(xj, l, 64);
+    if (lane == 0) {
+      cand[wave].nan = wnan >= 0 ? wave * 64 + wnan : -1;
+      cand[wave].idx = bm ? wave * 64 + l : -1;
+      cand[wave].best = m;
+      cand[wave].val = v;
+    }
+  }
+  __syncthreads();
+  // ---- combine the waves in row order: a NaN restarts the scan ----
+  int nanr = -1, ipiv = -1;
+  double best = -1.0, dtmp = 0.0;
+#pragma unroll
+  for (int w = 0; w < kUnbThreads / 64; ++w) {
+    const SaCand cw = cand[w];
+    if (cw.nan >= 0) {
+      nanr = cw.nan;
+      ipiv = cw.idx;
+      best = cw.best;
+      dtmp = cw.val;
+    } else if (cw.idx >= 0 && (ipiv < 0 || cw.best > best)) {
+      ipiv = cw.idx;
+      best = cw.best;
+      dtmp = cw.val;
+    }
+  }
+  if (ipiv < 0) {  // the column ends with a NaN: it is the choice
+    ipiv = nanr;
+    dtmp = __builtin_nan("");
+  }
+  const bool swap = fabs(ltmp) < fabs(dtmp);
+  if (swap && t == ipiv) {
+#pragma unroll
+    for (int c = 0; c < BSMAX; ++c) {
+      if (c < bs) {
+        const double a = Ls[J * LS + c];
+        Ls[J * LS + c] = x[c];
+        x[c] = a;
+      }
+    }
+  }
+  if (t == 0) p[J] = swap ? ipiv + bs - J : 0;
+  __syncthreads();
+  // ---- scale column J, rank-1 update of the columns right of it ----
+  const double dinv = 1.0 / Ls[J * LS + J];
+  if (act) {
+    x[J] = dinv * x[J];
+#pragma unroll
+    for (int c = J + 1; c < BSMAX; ++c) {
+      if (c < bs) {
+        const double y = Ls[J * LS + c];
+        if (y != 0.0) x[c] = __dadd_rn(x[c], __dmul_rn(x[J], -1.0 * y));
+      }
+    }
+  }
+  if (t >= J && t < bs) U[J + (long)t * ldu] = Ls[J * LS + t];
+}
+
+template <int BSMAX, int... J>
+__device__ __forceinline__ void sa_reg_columns(std::integer_sequence<int, J...>,
+                                               double (&x)[BSMAX], int bs, int t, bool act,
+                                               double* Ls, SaCand* cand, double* __restrict__ U,
+                                               long ldu, int* __restrict__ p) {
+  (sa_reg_column<BSMAX, J>(x, bs, t, act, Ls, cand, U, ldu, p), ...);
+}
+
+template <int BSMAX>
+__global__ __launch_bounds__(kUnbThreads) void sa_lu_reg_kernel(int bs, double* __restrict__ U,
+                                                                long ldu, int mD,
+                                                                double* __restrict__ D, long ldd,
+                                                                int* __restrict__ p,
+                                                                double* __restrict__ L,
+                                                                long ldl) {
+  constexpr int LS = BSMAX + 1;
+  __shared__ double Ls[BSMAX * LS];  // Ls[i * LS + c] = L(i, c)
+  __shared__ SaCand cand[kUnbThreads / 64];
+  const int t = threadIdx.x;
+  const bool act = t < mD;
+  double x[BSMAX];
+#pragma unroll
+  for (int c = 0; c < BSMAX; ++c) x[c] = (act && c < bs) ? D[t + (long)c * ldd] : 0.0;
+  for (int e = t; e < bs * bs; e += kUnbThreads) {
+    const int i = e % bs, c = e / bs;
+    Ls[i * LS + c] = (i <= c) ? U[i + c * ldu] : 0.0;
+  }
+  __syncthreads();
+  sa_reg_columns<BSMAX>(std::make_integer_sequence<int, BSMAX>{}, x, bs, t, act, Ls, cand, U, ldu,
+                        p);
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < BSMAX; ++c)
+      if (c < bs) D[t + (long)c * ldd] = x[c];
+  }
+  __syncthreads();
+  for (int e = t; e < bs * bs; e += kUnbThreads) {
+    const int i = e % bs, c = e / bs;
+    L[i + c * ldl] = Ls[i * LS + c];
+  }
+}
+
+int launch_sa_lu_unb(hipStream_t s, int bs, double* U, long ldu, int mD, double* D, long ldd,
+                     int* p, double* L, long ldl) {
+  if (mD <= kUnbThreads && bs <= 32)
+    hipLaunchKernelGGL(sa_lu_reg_kernel<32>, dim3(1), dim3(kUnbThreads), 0, s, bs, U, ldu, mD, D,
+                       ldd, p, L, ldl);
+  else if (mD <= kUnbThreads && bs <= 64)
+    hipLaunchKernelGGL(sa_lu_reg_kernel<64>, dim3(1), dim3(kUnbThreads), 0, s, bs, U, ldu, mD, D,
+                       ldd, p, L, ldl);
+  else
+    hipLaunchKernelGGL(sa_lu_unb_kernel, dim3(1), dim3(kUnbThreads), 0, s, bs, U, ldu, mD, D, ldd,
+                       p, L, ldl);
+  FLA_HIP_CHECK(hipGetLastError());
+  return FLA_HIP_OK;
+}
+
 // FLA_SA_Apply_pivots: for i in order, row i of C (bs rows) <-> row
 // p[i] - (bs - i) of E when p[i] != 0; one thread per column.
 __global__ void sa_swap_kernel(int bs, int ncols, double* __restrict__ C, long ldc,
@@ -191,9 +350,7 @@ int sa_lu(hipStream_t s, int b, int nb_alg, double* U, long ldu, double* D, long
     double* U12 = U + s0 + (long)(s0 + bs) * ldu;
     double* D1 = D + (long)s0 * ldd;
     double* D2 = D + (long)(s0 + bs) * ldd;
-    hipLaunchKernelGGL(sa_lu_unb_kernel, dim3(1), dim3(kUnbThreads), 0, s, bs, U11, ldu, b, D1,
-                       ldd, p + s0, L + s0, ldl);
-    FLA_HIP_CHECK(hipGetLastError());
+    FLA_TRY(launch_sa_lu_unb(s, bs, U11, ldu, b, D1, ldd, p + s0, L + s0, ldl));
     if (nrest > 0) {
       FLA_TRY(launch_sa_swap(s, bs, nrest, U12, ldu, D2, ldd, p + s0));
       FLA_TRY(unit_lower_solve(s, bs, nrest, L + s0, ldl, U12, ldu));

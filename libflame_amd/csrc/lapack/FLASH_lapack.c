@@ -168,6 +168,36 @@ dim_t FLASH_LU_incpiv_determine_alg_blocksize(FLA_Obj A) {
   return (dim_t)(b > 1.0 ? b : 1.0);
 }
 
+/* Zeroes the one device allocation behind a freshly created depth-1
+   hierarchy (its leaves are consecutive, starting at the top-left leaf). */
+static FLA_Error zero_leaves(FLA_Obj H) {
+  const size_t count = (size_t)FLASH_Obj_scalar_length(H) * (size_t)FLASH_Obj_scalar_width(H);
+  void* buf;
+  int st;
+  if (count == 0) return FLA_SUCCESS;
+  buf = FLA_Obj_buffer_at_view(*FLASH_OBJ_PTR_AT(H));
+  if (FLA_Obj_datatype(H) == FLA_DOUBLE && count <= 0x7fffffffu) {
+    st = fla_hip_dlaset(0, (int)count, 1, 0.0, 0.0, (double*)buf, (int)count);
+    if (st == FLA_HIP_OK) st = fla_hip_synchronize();
+  } else {
+    const size_t bytes = count * (size_t)FLA_Obj_elem_size(*FLASH_OBJ_PTR_AT(H));
+    void* z = calloc(bytes, 1);
+    if (!z) return FLA_FAILURE;
+    st = fla_hip_memcpy_h2d(buf, z, bytes);
+    free(z);
+  }
+  if (st != FLA_HIP_OK) {
+    fprintf(stderr, "libflame_amd: FLASH_LU_incpiv_create_hier_matrices: %s\n",
+            fla_hip_error_string(st));
+    return FLA_GPU_ERROR;
+  }
+  return FLA_SUCCESS;
+}
+
+/* p and L are zeroed on creation: the reference leaves them uninitialised
+   (FLASH_Obj_create_ext), and the parts of L that the tile algorithm never
+   writes (beyond a ragged last inner block) would otherwise hold whatever
+   the allocation held before. */
 FLA_Error FLASH_LU_incpiv_create_hier_matrices(FLA_Obj A_flat, dim_t depth, dim_t* b_flash,
                                                dim_t b_alg, FLA_Obj* A, FLA_Obj* p, FLA_Obj* L) {
   dim_t one = 1, m, n;
@@ -183,8 +213,11 @@ FLA_Error FLASH_LU_incpiv_create_hier_matrices(FLA_Obj A_flat, dim_t depth, dim_
   if ((e = FLASH_Obj_create_ext(FLA_INT, m * b_flash[0], n, depth, b_flash, &one, p)) !=
       FLA_SUCCESS)
     return e;
-  return FLASH_Obj_create_ext(FLA_Obj_datatype(A_flat), m * b_flash[0], n * b_alg, depth, b_flash,
-                              &b_alg, L);
+  if ((e = zero_leaves(*p)) != FLA_SUCCESS) return e;
+  if ((e = FLASH_Obj_create_ext(FLA_Obj_datatype(A_flat), m * b_flash[0], n * b_alg, depth,
+                                b_flash, &b_alg, L)) != FLA_SUCCESS)
+    return e;
+  return zero_leaves(*L);
 }
 
 static FLA_Error incpiv_checks(FLA_Obj A, FLA_Obj p, FLA_Obj L, const char* fn) {
