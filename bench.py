@@ -729,7 +729,7 @@ def main():
                        "lu_roofline": {"bound": "mfma", "achieved": round(upd_tf, 3),
                                        "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
                                        "frac": round(upd_tf / PEAK_FP64_MFMA, 4),
-                                       "kernel": "dgemm_mfma_kernel<false,true> (LU A22 update)",
+                                       "kernel": "dgemm_nt_kp_kernel (LU A22 update, U12 transposed)",
                                        "launches": gn // nrep}})
         log(f"LU: update {upd_tf:.2f} TFLOP/s, panels {pms / nrep:.1f} ms, "
             f"interchanges {sms / nrep:.1f} ms ({swap_bytes / 1e9:.2f} GB)")
@@ -768,7 +768,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
                      "traffic": profiled_traffic(),
-                     "kernel": "dgemm_mfma_kernel<false,true> (lower SYRK update)",
+                     "kernel": "dgemm_nt_kp_kernel (lower SYRK update)",
                      "launches": upd_n},
         "cpu_baseline": cpu,
         "extras": extras,

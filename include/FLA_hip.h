@@ -175,7 +175,8 @@ int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb);
 /* Outer Cholesky block used for order n (chol_nb 0 = automatic: n/32
    rounded to a multiple of 128 within [256, 1024]). */
 int fla_hip_chol_blocksize(int n);
-/* Tile configuration of the MFMA update kernel (0 = default). */
+/* Tile configuration of the MFMA update kernel (0 = default, 1..15 the
+   measured alternatives; anything else is an argument error). */
 int fla_hip_set_gemm_config(int cfg);
 /* Width G (tile columns) of the groups in which update tiles are ordered, so
    that concurrently running tiles share operand panels in L2 (default 4). */

@@ -169,6 +169,12 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
   __shared__ double prow[PW];                // pivot row values
   __shared__ double jrow[PW];                // old diagonal row values
   __shared__ int sel[4];                     // pivot row, its orig, orig of the old row j
+  // The panel's pivots and its first null pivot, written to global memory
+  // after the column loop: a global store inside the loop would hold the
+  // storing wave at the next column's wait for outstanding memory operations
+  // until the store is acknowledged, and every workgroup waits on workgroup 0.
+  __shared__ int pivs[PW];
+  __shared__ int first_null;
   __shared__ double stage[NT / 64][32];      // a row's 32 columns, per wave, to store coalesced
 
   if (wg == 0 && t == 0) {
@@ -176,6 +182,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     // workgroup's first header, which is stored after this reset completes.
     __hip_atomic_store(&moved[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
+    first_null = -1;
   }
 
   // Thread owns local rows rh*128 + 64*q + lane (q = 0..1), columns 32*cg + c.
@@ -450,8 +457,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
     const bool null_pivot = (piv_val == 0.0);
     const bool swap = !null_pivot && r != j;
     if (wg == 0 && t == 0) {
-      piv[k + j] = r - j;
-      if (null_pivot) atomicCAS(info, -1, k + j);
+      pivs[j] = r - j;
+      if (null_pivot && first_null < 0) first_null = j;
     }
     // ---- interchange rows j and r (skipped for a null pivot) ----
     if (swap) {
@@ -530,6 +537,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(int M, int W, double* __re
   }
   __syncthreads();
 
+  if (wg == 0) {
+    for (int jj = t; jj < W; jj += NT) piv[k + jj] = pivs[jj];
+    if (t == 0 && first_null >= 0) atomicCAS(info, -1, k + first_null);
+  }
   // ---- write back; record rows whose content moved ----
 #pragma unroll
   for (int q = 0; q < RQ; ++q) {
