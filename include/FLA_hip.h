@@ -44,6 +44,19 @@ extern "C" {
 int fla_hip_init(void);     /* selects device 0, creates the two streams */
 int fla_hip_finalize(void); /* releases streams, events and workspaces */
 int fla_hip_synchronize(void);
+/* The caller's stream (a hipStream_t; NULL: none, the default).  Replaces
+   the per-call rocblas_handle, and the stream it carries, that the
+   reference's FLA_*_external_hip wrappers take
+   (src/base/flamec/supermatrix/hip/main/FLASH_Queue_hip.c:297-318).  While
+   one is set, every entry point that queues device work orders that work
+   after the work already queued on the caller's stream.  It also orders the
+   caller's later work on that stream after its own, so asynchronous entry
+   points (dgemm, dlacpy, ...) compose with the caller's kernels without a
+   host synchronisation.  The library keeps its own streams; the ordering
+   uses events created at initialisation.  fla_hip_set_stream initialises
+   the library if needed; FLA_Finalize / fla_hip_finalize clear the setting. */
+int fla_hip_set_stream(void* stream);
+void* fla_hip_get_stream(void);
 int fla_hip_malloc(void** ptr, size_t bytes);
 int fla_hip_free(void* ptr);
 int fla_hip_memcpy_h2d(void* dst, const void* src, size_t bytes);
@@ -175,7 +188,7 @@ int fla_hip_get_blocksizes(int* chol_nb, int* lu_nb);
 /* Outer Cholesky block used for order n (chol_nb 0 = automatic: n/32
    rounded to a multiple of 128 within [256, 1024]). */
 int fla_hip_chol_blocksize(int n);
-/* Tile configuration of the MFMA update kernel (0 = default, 1..15 the
+/* Tile configuration of the MFMA update kernel (0 = default, 1..19 the
    measured alternatives; anything else is an argument error). */
 int fla_hip_set_gemm_config(int cfg);
 /* Width G (tile columns) of the groups in which update tiles are ordered, so

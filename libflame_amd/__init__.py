@@ -153,6 +153,8 @@ def lib():
                                             C.c_int, C.c_int]),
             "fla_hip_init": (C.c_int, []), "fla_hip_finalize": (C.c_int, []),
             "fla_hip_synchronize": (C.c_int, []),
+            "fla_hip_set_stream": (C.c_int, [vp]),
+            "fla_hip_get_stream": (vp, []),
             "fla_hip_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
             "fla_hip_free": (C.c_int, [vp]),
             "fla_hip_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),

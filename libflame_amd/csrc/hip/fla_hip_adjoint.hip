@@ -73,6 +73,7 @@ inline unsigned nblk(long total) { return (unsigned)((total + 255) / 256); }
 
 extern "C" int fla_hip_dchol_adjoint(int n, const double* L, int ldl, const double* Lbar,
                                      int ldlb, double* Abar, int ldab) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n < 0 || ldl < (n > 1 ? n : 1) || ldlb < (n > 1 ? n : 1) || ldab < (n > 1 ? n : 1))
     return FLA_HIP_EARG;
@@ -108,6 +109,7 @@ extern "C" int fla_hip_dchol_solve_adjoint(int n, int nrhs, const double* L, int
                                            const double* X, int ldx, const double* Xbar,
                                            int ldxb, double* Bbar, int ldbb, double* Abar,
                                            int ldab) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   const int m1 = n > 1 ? n : 1;
   if (n < 0 || nrhs < 0 || ldl < m1 || ldx < m1 || ldxb < m1 || ldbb < m1 || ldab < m1)
@@ -236,6 +238,7 @@ __global__ void logdet_bar_kernel(int n, const double* __restrict__ L, long ldl,
 
 extern "C" int fla_hip_gp_rbf_kernel(int n, int d, const double* x, int ldx, double l, double s,
                                      double noise, double* K, int ldk) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n < 0 || d < 1 || ldx < (n > 1 ? n : 1) || ldk < (n > 1 ? n : 1) || !(l > 0.0))
     return FLA_HIP_EARG;
@@ -248,6 +251,7 @@ extern "C" int fla_hip_gp_rbf_kernel(int n, int d, const double* x, int ldx, dou
 
 extern "C" int fla_hip_gp_rbf_grad(int n, int d, const double* x, int ldx, double l, double s,
                                    const double* Kbar, int ldkb, double* g) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n < 1 || d < 1 || ldx < n || ldkb < n || !(l > 0.0) || !g) return FLA_HIP_EARG;
   hipStream_t st = fla_ctx().s_update;
@@ -274,6 +278,7 @@ extern "C" int fla_hip_gp_rbf_grad(int n, int d, const double* x, int ldx, doubl
 extern "C" int fla_hip_gp_rbf_lml_grad(int n, int d, const double* x, int ldx, const double* y,
                                        double l, double s, double noise, double* lml,
                                        double* grad, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n < 1 || d < 1 || ldx < n || !lml || !grad || !info) return FLA_HIP_EARG;
   hipStream_t st = fla_ctx().s_update;

@@ -308,6 +308,7 @@ __global__ void copy_rows_kernel(int count, int n, const float* __restrict__ y,
 
 extern "C" int fla_hip_schol_adjoint_batched(int count, int n, const float* L, const float* Lbar,
                                              float* Abar) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (count < 0 || n < 0 || n > TN) return FLA_HIP_EARG;
   if (count == 0 || n == 0) return FLA_HIP_OK;
@@ -321,6 +322,7 @@ extern "C" int fla_hip_schol_adjoint_batched(int count, int n, const float* L, c
 extern "C" int fla_hip_sgp_rbf_lml_grad_batched(int count, int n, int d, const float* x,
                                                 const float* y, const float* l, float s,
                                                 float noise, float* lml, float* grad, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (count < 0 || n < 1 || n > TN || d < 1 || d > 8) return FLA_HIP_EARG;
   if (count == 0) return FLA_HIP_OK;

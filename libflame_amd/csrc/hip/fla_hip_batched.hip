@@ -687,12 +687,14 @@ __global__ void tune_fill_kernel(int n, double* A, float* Af, int spd, unsigned 
 extern "C" int fla_hip_dposv_batched(int count, const int* n, double* const* A, const int* lda,
                                      double* const* B, const int* ldb, const int* nrhs,
                                      int* info) {
+  FLA_USER_STREAM_SCOPE;
   return batched_entry(kDpotrf, count, n, reinterpret_cast<void* const*>(A), lda, nullptr,
                        reinterpret_cast<void* const*>(B), ldb, nrhs, info);
 }
 
 extern "C" int fla_hip_sposv_batched(int count, const int* n, float* const* A, const int* lda,
                                      float* const* B, const int* ldb, const int* nrhs, int* info) {
+  FLA_USER_STREAM_SCOPE;
   return batched_entry(kSpotrf, count, n, reinterpret_cast<void* const*>(A), lda, nullptr,
                        reinterpret_cast<void* const*>(B), ldb, nrhs, info);
 }
@@ -700,6 +702,7 @@ extern "C" int fla_hip_sposv_batched(int count, const int* n, float* const* A, c
 extern "C" int fla_hip_dgesv_batched(int count, const int* n, double* const* A, const int* lda,
                                      int* const* ipiv, double* const* B, const int* ldb,
                                      const int* nrhs, int* info) {
+  FLA_USER_STREAM_SCOPE;
   return batched_entry(kDgetrf, count, n, reinterpret_cast<void* const*>(A), lda, ipiv,
                        reinterpret_cast<void* const*>(B), ldb, nrhs, info);
 }
@@ -744,6 +747,7 @@ extern "C" int fla_hip_batched_set_config(int kind, int nb, int wg_per_cu) {
 // fastest for this batch's size signature.  Returns the choice.
 extern "C" int fla_hip_batched_tune(int kind, int count, const int* n, const int* nrhs,
                                     int* best_nb, int* best_wg, double* best_ms) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (kind < 0 || kind >= kKinds || count <= 0 || !n) return FLA_HIP_EARG;
   hipStream_t s = fla_ctx().s_update;

@@ -441,6 +441,7 @@ static int chol_lower(int n, double* A, long lda, int* dinfo) {
 }
 
 extern "C" int fla_hip_dpotrf(int uplo, int n, double* A, int lda, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
   if (n < 0 || lda < (n > 1 ? n : 1)) return FLA_HIP_EARG;

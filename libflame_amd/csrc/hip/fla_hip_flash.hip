@@ -40,6 +40,7 @@ __global__ __launch_bounds__(kCopyThreads) void tile_copy_kernel(const long long
 
 extern "C" int fla_hip_tile_copy(int ntiles, const long long* desc, void* flat, long ldf,
                                  int elem_bytes, int to_tiles) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (ntiles < 0 || (elem_bytes != 4 && elem_bytes != 8) || ldf < 1) return FLA_HIP_EARG;
   if (ntiles == 0) return FLA_HIP_OK;

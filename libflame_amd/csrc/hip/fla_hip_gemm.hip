@@ -239,17 +239,19 @@ __attribute__((amdgpu_waves_per_eu(WM * WN >= 8 ? 4 : 2))) void dgemm_mfma_kerne
 // (half the LDS reads and writes of the 8-byte layout; the sum over a step
 // pair covers k = 8p .. 8p+7 once).  The loads need no bounds guards: the
 // chunk's base is uniform and each lane adds a 32-bit offset.
-constexpr int KP_BM = 128, KP_BK = 16, KP_LD = KP_BM + 4;  // LD in 16-byte entries
+constexpr int KP_BM = 128, KP_BK = 16;
 typedef double kp_d2 __attribute__((ext_vector_type(2)));
 
-// BK: K chunk (a multiple of 16); WM x WN waves, each (128/WM) x (128/WN).
-template <bool VEC, int BK, int WM, int WN>
+// BK: K chunk (a multiple of 16); WM x WN waves, each (128/WM) x (128/WN);
+// PAD: 16-byte entries between the LDS rows of consecutive k pairs.
+template <bool VEC, int BK, int WM, int WN, int PAD>
 __global__ __launch_bounds__(64 * WM * WN)
 __attribute__((amdgpu_waves_per_eu((WM * WN >= 8 && BK <= 16) ? 4 : 2))) void dgemm_nt_kp_kernel(
     int mode, int m, int n, int k, double alpha, const double* A, long lda, const double* B,
     long ldb, double beta, double* C, long ldc, int Tm, int Tn, int rpc,
     const int* __restrict__ abort_flag) {
   constexpr int NT = 64 * WM * WN, TM = KP_BM / WM / 16, TN = KP_BM / WN / 16, NKP = BK / 2;
+  constexpr int KP_LD = KP_BM + PAD;  // LDS row of one k pair, in 16-byte entries
   // VEC: 2 x 2 blocks (two rows, one k pair) per thread; else single k pairs
   constexpr int NREG = VEC ? 2 * (64 * NKP / NT) : KP_BM * NKP / NT;
   static_assert(!VEC || (64 * NKP) % NT == 0, "whole 2 x 2 blocks per thread");
@@ -353,7 +355,7 @@ __attribute__((amdgpu_waves_per_eu((WM * WN >= 8 && BK <= 16) ? 4 : 2))) void dg
                                            li, lk);
 }
 
-template <int BK, int WM, int WN>
+template <int BK, int WM, int WN, int PAD = 4>
 int launch_kp(hipStream_t s, bool vec, int mode, int m, int n, int k, double alpha,
               const double* A, long lda, const double* B, long ldb, double beta, double* C,
               long ldc, const int* abort_flag) {
@@ -362,11 +364,11 @@ int launch_kp(hipStream_t s, bool vec, int mode, int m, int n, int k, double alp
   if (nblk > 0x7fffffffL) return FLA_HIP_EARG;
   constexpr bool vec_ok = (64 * (BK / 2)) % (64 * WM * WN) == 0;
   if (vec && vec_ok)
-    hipLaunchKernelGGL((dgemm_nt_kp_kernel<vec_ok, BK, WM, WN>), dim3((unsigned)nblk),
+    hipLaunchKernelGGL((dgemm_nt_kp_kernel<vec_ok, BK, WM, WN, PAD>), dim3((unsigned)nblk),
                        dim3(64 * WM * WN), 0, s, mode, m, n, k, alpha, A, lda, B, ldb, beta, C,
                        ldc, Tm, Tn, g_tile_group, abort_flag);
   else
-    hipLaunchKernelGGL((dgemm_nt_kp_kernel<false, BK, WM, WN>), dim3((unsigned)nblk),
+    hipLaunchKernelGGL((dgemm_nt_kp_kernel<false, BK, WM, WN, PAD>), dim3((unsigned)nblk),
                        dim3(64 * WM * WN), 0, s, mode, m, n, k, alpha, A, lda, B, ldb, beta, C,
                        ldc, Tm, Tn, g_tile_group, abort_flag);
   FLA_HIP_CHECK(hipGetLastError());
@@ -414,7 +416,7 @@ extern "C" int fla_hip_set_gemm_group(int G) {
 }
 
 extern "C" int fla_hip_set_gemm_config(int cfg) {
-  if (cfg < 0 || cfg > 15) return FLA_HIP_EARG;
+  if (cfg < 0 || cfg > 19) return FLA_HIP_EARG;
   g_gemm_cfg = cfg;
   return FLA_HIP_OK;
 }
@@ -446,6 +448,18 @@ int launch_dgemm_mfma(hipStream_t s, bool ta, bool tb, int mode, int m, int n, i
         case 15:
           return launch_kp<32, 4, 2>(s, vec, mode, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                                      abort_flag);
+        case 16:
+          return launch_kp<16, 4, 2, 0>(s, vec, mode, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                        ldc, abort_flag);
+        case 17:
+          return launch_kp<16, 4, 2, 1>(s, vec, mode, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                        ldc, abort_flag);
+        case 18:
+          return launch_kp<16, 4, 2, 2>(s, vec, mode, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                        ldc, abort_flag);
+        case 19:
+          return launch_kp<16, 4, 2, 8>(s, vec, mode, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                        ldc, abort_flag);
         default:
           return launch_kp<16, 4, 2>(s, vec, mode, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                                      abort_flag);
@@ -505,6 +519,7 @@ static bool is_trans(int t) { return t == kTrans || t == kConjTrans; }
 extern "C" int fla_hip_dgemm(int transa, int transb, int m, int n, int k, double alpha,
                              const double* A, int lda, const double* B, int ldb, double beta,
                              double* C, int ldc) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m < 0 || n < 0 || k < 0) return FLA_HIP_EARG;
   if (m == 0 || n == 0) return FLA_HIP_OK;
@@ -524,6 +539,7 @@ extern "C" int fla_hip_dgemm(int transa, int transb, int m, int n, int k, double
 
 extern "C" int fla_hip_dsyrk(int uplo, int trans, int n, int k, double alpha,
                              const double* A, int lda, double beta, double* C, int ldc) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n < 0 || k < 0) return FLA_HIP_EARG;
   if (n == 0) return FLA_HIP_OK;

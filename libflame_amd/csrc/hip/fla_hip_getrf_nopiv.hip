@@ -81,6 +81,7 @@ __global__ __launch_bounds__(256) void transpose_nopiv_kernel(int r, int c,
 }  // namespace
 
 extern "C" int fla_hip_dgetrf_nopiv(int m, int n, double* A, int lda, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
   *info = kSuccess;

@@ -378,6 +378,7 @@ int incpiv_args(int m, int n, int lda, int b, int nb_alg, int ldp, int ldl) {
 
 extern "C" int fla_hip_dgetrf_incpiv(int m, int n, double* A, int lda, int b, int nb_alg, int* p,
                                      int ldp, double* L, int ldl) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   FLA_TRY(incpiv_args(m, n, lda, b, nb_alg, ldp, ldl));
   const int mt = m / b, nt = n / b, kt = mt < nt ? mt : nt;
@@ -412,6 +413,7 @@ extern "C" int fla_hip_dgetrf_incpiv(int m, int n, double* A, int lda, int b, in
 extern "C" int fla_hip_dfs_incpiv(int m, int n, const double* A, int lda, int b, int nb_alg,
                                   const int* p, int ldp, const double* L, int ldl, int nrhs,
                                   double* X, int ldx) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   FLA_TRY(incpiv_args(m, n, lda, b, nb_alg, ldp, ldl));
   if (nrhs < 0 || ldx < (m > 1 ? m : 1)) return FLA_HIP_EARG;

@@ -39,6 +39,11 @@ struct FlaHipCtx {
   hipStream_t s_update_lp = nullptr;
   static constexpr int kEvents = 8;
   hipEvent_t ev[kEvents] = {};
+  // The caller's stream (fla_hip_set_stream; null: none) and the events that
+  // order the library's three streams after it on entry and it after them on
+  // return.
+  hipStream_t user_stream = nullptr;
+  hipEvent_t ev_user[4] = {};
   // Grow-only device workspaces: slot 0 general scratch (dtrsm, dlaswp),
   // slot 1 the mixed-precision solver's buffers, slot 2 its solve scratch,
   // slot 3 generators and small reductions, slot 4 the hierarchical-storage
@@ -73,6 +78,18 @@ FlaHipCtx& fla_ctx();
 // larger blocks raise the trailing update's arithmetic intensity).
 int fla_chol_nb(int n);
 int fla_ensure_ready();
+// Orders a public entry point's device work after the work already queued on
+// the caller's stream (on construction) and that stream's later work after
+// the entry point's (on destruction), when a caller's stream is set.  Uses
+// events created at initialisation only: no allocation.
+struct FlaUserStreamScope {
+  FlaUserStreamScope();
+  ~FlaUserStreamScope();
+  FlaUserStreamScope(const FlaUserStreamScope&) = delete;
+  FlaUserStreamScope& operator=(const FlaUserStreamScope&) = delete;
+};
+#define FLA_USER_STREAM_SCOPE FlaUserStreamScope fla_user_stream_scope_
+
 // Returns a device workspace of at least `bytes` bytes (grow-only).
 int fla_workspace(size_t bytes, void** ptr);
 int fla_workspace_slot(int slot, size_t bytes, void** ptr);

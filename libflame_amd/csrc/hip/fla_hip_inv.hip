@@ -112,6 +112,7 @@ static int ttmm_op(hipStream_t s, int n, double* A, long lda, bool, double* Z) {
 }
 
 extern "C" int fla_hip_dtrtri(int uplo, int diag, int n, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if ((uplo != kLower && uplo != kUpper) || (diag != kUnit && diag != kNonUnit)) return FLA_HIP_EARG;
   if (n < 0 || lda < (n > 1 ? n : 1)) return FLA_HIP_EARG;
@@ -120,6 +121,7 @@ extern "C" int fla_hip_dtrtri(int uplo, int diag, int n, double* A, int lda) {
 }
 
 extern "C" int fla_hip_dlauum(int uplo, int n, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
   if (n < 0 || lda < (n > 1 ? n : 1)) return FLA_HIP_EARG;
@@ -128,6 +130,7 @@ extern "C" int fla_hip_dlauum(int uplo, int n, double* A, int lda) {
 }
 
 extern "C" int fla_hip_dspdinv(int uplo, int n, double* A, int lda, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_hip_dpotrf(uplo, n, A, lda, info));
   if (*info != kSuccess || n == 0) return FLA_HIP_OK;
   FLA_TRY(fla_hip_dtrtri(uplo, kNonUnit, n, A, lda));

@@ -897,6 +897,7 @@ static double g_mp_stats[8];
 
 extern "C" int fla_hip_mp_chol(int uplo, int n, const double* A, int lda, int precision,
                                double* L, int ldl, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (uplo != kLower && uplo != kUpper) return FLA_HIP_EARG;
   if (precision != FLA_MP_FP32 && precision != FLA_MP_BF16) return FLA_HIP_EARG;
@@ -1169,17 +1170,20 @@ static int mp_solve(bool general, int uplo, int n, int nrhs, double* A, int lda,
 extern "C" int fla_hip_dsposv(int uplo, int n, int nrhs, double* A, int lda, const double* B,
                               int ldb, double* X, int ldx, const fla_hip_mp_opts* opts, int* iter,
                               int* info) {
+  FLA_USER_STREAM_SCOPE;
   return mp_solve(false, uplo, n, nrhs, A, lda, nullptr, B, ldb, X, ldx, opts, iter, info);
 }
 
 extern "C" int fla_hip_dsgesv(int n, int nrhs, double* A, int lda, int* p, const double* B,
                               int ldb, double* X, int ldx, const fla_hip_mp_opts* opts, int* iter,
                               int* info) {
+  FLA_USER_STREAM_SCOPE;
   return mp_solve(true, kLower, n, nrhs, A, lda, p, B, ldb, X, ldx, opts, iter, info);
 }
 
 extern "C" int fla_hip_mp_lu(int n, const double* A, int lda, int precision, double* F, int ldf,
                              int* p, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (precision != FLA_MP_FP32 && precision != FLA_MP_BF16) return FLA_HIP_EARG;
   if (n < 0 || lda < (n > 1 ? n : 1) || ldf < (n > 1 ? n : 1)) return FLA_HIP_EARG;

@@ -93,6 +93,8 @@ extern "C" int fla_hip_init(void) {
   FLA_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.s_panel, hipStreamNonBlocking));
   for (int i = 0; i < FlaHipCtx::kEvents; ++i)
     FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
+  for (int i = 0; i < 4; ++i)
+    FLA_HIP_CHECK(hipEventCreateWithFlags(&g_ctx.ev_user[i], hipEventDisableTiming));
   FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dscratch), 4096));
   FLA_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&g_ctx.dlinv), 128 * 128 * sizeof(double)));
   // uncached exchange memory: allocated once per process, never freed
@@ -128,11 +130,40 @@ extern "C" int fla_hip_finalize(void) {
   if (g_ctx.dlinv) FLA_HIP_CHECK(hipFree(g_ctx.dlinv));
   g_ctx.dlinv = nullptr;
   for (int i = 0; i < FlaHipCtx::kEvents; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev[i]));
+  for (int i = 0; i < 4; ++i) FLA_HIP_CHECK(hipEventDestroy(g_ctx.ev_user[i]));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_update_lp));
   FLA_HIP_CHECK(hipStreamDestroy(g_ctx.s_panel));
+  g_ctx.user_stream = nullptr;
   g_ctx.ready = false;
   return FLA_HIP_OK;
+}
+
+// ---- the caller's stream -------------------------------------------------
+extern "C" int fla_hip_set_stream(void* stream) {
+  FLA_TRY(fla_ensure_ready());
+  g_ctx.user_stream = static_cast<hipStream_t>(stream);
+  return FLA_HIP_OK;
+}
+
+extern "C" void* fla_hip_get_stream(void) { return g_ctx.user_stream; }
+
+FlaUserStreamScope::FlaUserStreamScope() {
+  if (!g_ctx.ready || !g_ctx.user_stream) return;
+  // errors here resurface at the entry point's own calls on the same streams
+  if (hipEventRecord(g_ctx.ev_user[0], g_ctx.user_stream) != hipSuccess) return;
+  (void)hipStreamWaitEvent(g_ctx.s_update, g_ctx.ev_user[0], 0);
+  (void)hipStreamWaitEvent(g_ctx.s_panel, g_ctx.ev_user[0], 0);
+  (void)hipStreamWaitEvent(g_ctx.s_update_lp, g_ctx.ev_user[0], 0);
+}
+
+FlaUserStreamScope::~FlaUserStreamScope() {
+  if (!g_ctx.ready || !g_ctx.user_stream) return;
+  const hipStream_t mine[3] = {g_ctx.s_update, g_ctx.s_panel, g_ctx.s_update_lp};
+  for (int i = 0; i < 3; ++i) {
+    if (hipEventRecord(g_ctx.ev_user[1 + i], mine[i]) == hipSuccess)
+      (void)hipStreamWaitEvent(g_ctx.user_stream, g_ctx.ev_user[1 + i], 0);
+  }
 }
 
 extern "C" int fla_hip_synchronize(void) {
@@ -187,12 +218,15 @@ static int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kin
 }
 
 extern "C" int fla_hip_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  FLA_USER_STREAM_SCOPE;
   return copy_sync(dst, src, bytes, hipMemcpyHostToDevice);
 }
 extern "C" int fla_hip_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  FLA_USER_STREAM_SCOPE;
   return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost);
 }
 extern "C" int fla_hip_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  FLA_USER_STREAM_SCOPE;
   return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
@@ -210,19 +244,23 @@ static int copy2d(int m, int n, const double* s, int lds, double* d, int ldd,
 
 extern "C" int fla_hip_dmatrix_h2d(int m, int n, const double* h, int ldh, double* d,
                                    int ldd) {
+  FLA_USER_STREAM_SCOPE;
   return copy2d(m, n, h, ldh, d, ldd, hipMemcpyHostToDevice);
 }
 extern "C" int fla_hip_dmatrix_d2h(int m, int n, const double* d, int ldd, double* h,
                                    int ldh) {
+  FLA_USER_STREAM_SCOPE;
   return copy2d(m, n, d, ldd, h, ldh, hipMemcpyDeviceToHost);
 }
 extern "C" int fla_hip_dmatrix_d2d(int m, int n, const double* s, int lds, double* d,
                                    int ldd) {
+  FLA_USER_STREAM_SCOPE;
   return copy2d(m, n, s, lds, d, ldd, hipMemcpyDeviceToDevice);
 }
 
 extern "C" int fla_hip_memcpy2d(void* dst, size_t dpitch, const void* src, size_t spitch,
                                 size_t width, size_t height, int kind) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (width == 0 || height == 0) return FLA_HIP_OK;
   const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice

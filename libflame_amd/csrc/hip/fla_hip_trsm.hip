@@ -531,6 +531,7 @@ static int dtrsm_body(int side, int uplo, int trans, int diag, int m, int n, dou
 
 extern "C" int fla_hip_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double alpha,
                              const double* A, int lda, double* B, int ldb) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (!dtrsm_args_ok(side, uplo, m, n, lda, ldb)) return FLA_HIP_EARG;
   if (m == 0 || n == 0) return FLA_HIP_OK;
@@ -576,6 +577,7 @@ int fla_noalloc_setup(int device) {
 extern "C" int fla_hip_dtrsm_noalloc(int side, int uplo, int trans, int diag, int m, int n,
                                      double alpha, const double* A, int lda, double* B,
                                      int ldb) {
+  FLA_USER_STREAM_SCOPE;
   FlaHipCtx& c = fla_ctx();
   // no implicit initialization here: fla_hip_init allocates streams and buffers
   if (!c.ready || c.device < 0 || c.device >= kNoAllocDevices || !g_scratch_ptr[c.device]) {

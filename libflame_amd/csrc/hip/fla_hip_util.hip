@@ -307,6 +307,7 @@ __global__ void rank2_kernel(int n, const double* __restrict__ v, const double* 
 }  // namespace
 
 extern "C" int fla_hip_gen_uniform(unsigned long long seed, int m, int n, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m <= 0 || n <= 0) return FLA_HIP_OK;
   const long total = (long)m * n;
@@ -317,6 +318,7 @@ extern "C" int fla_hip_gen_uniform(unsigned long long seed, int m, int n, double
 }
 
 extern "C" int fla_hip_gen_normal(unsigned long long seed, int m, int n, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m <= 0 || n <= 0) return FLA_HIP_OK;
   const long total = (long)m * n;
@@ -328,6 +330,7 @@ extern "C" int fla_hip_gen_normal(unsigned long long seed, int m, int n, double*
 
 extern "C" int fla_hip_gram_lower(int n, int k, const double* B, int ldb, double scale,
                                   double shift, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n <= 0) return FLA_HIP_OK;
   hipStream_t s = fla_ctx().s_update;
@@ -343,6 +346,7 @@ extern "C" int fla_hip_gram_lower(int n, int k, const double* B, int ldb, double
 // generated matrix has the same bytes as a host generator's (the device pow
 // rounds differently in the last bit for some rows).
 extern "C" int fla_hip_scale_rows_logspace(int m, int n, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m <= 0 || n <= 0) return FLA_HIP_OK;
   std::vector<double> h((size_t)m);
@@ -363,6 +367,7 @@ extern "C" int fla_hip_scale_rows_logspace(int m, int n, double* A, int lda) {
 extern "C" int fla_hip_dmatmul_check(int kind, int uplo, int trans, int diag, int n, int nrhs,
                                      const double* M, int ldm, const double* X, int ldx,
                                      double beta, double* Y, int ldy) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n <= 0 || nrhs <= 0) return FLA_HIP_OK;
   if (nrhs > 8 || kind < 0 || kind > 2) return FLA_HIP_EARG;
@@ -380,6 +385,7 @@ extern "C" int fla_hip_dmatmul_check(int kind, int uplo, int trans, int diag, in
 }
 
 extern "C" int fla_hip_shift_pivots(int ptype, int m, int* p) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m <= 0) return FLA_HIP_OK;
   if (ptype != kNativePivots && ptype != kLapackPivots) return FLA_HIP_EARG;
@@ -428,6 +434,7 @@ int launch_laswp(hipStream_t s, int rows, int ncols, double* A, long lda, int k1
 
 extern "C" int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, const int* p,
                               int incp) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   // Every row index must lie inside the leading dimension.
   return launch_laswp(fla_ctx().s_update, lda, ncols, A, lda, k1, k2, p, incp);
@@ -435,6 +442,7 @@ extern "C" int fla_hip_dlaswp(int ncols, double* A, int lda, int k1, int k2, con
 
 extern "C" int fla_hip_dlaswp_rows(int m, int ncols, double* A, int lda, int k1, int k2,
                                    const int* p, int incp) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m > lda) return FLA_HIP_EARG;
   return launch_laswp(fla_ctx().s_update, m, ncols, A, lda, k1, k2, p, incp);
@@ -446,6 +454,7 @@ extern "C" int fla_hip_dlaswp_rows(int m, int ncols, double* A, int lda, int k1,
 // A := H A H = A - v z^T - z v^T.  Both triangles are written.
 extern "C" int fla_hip_gen_spd_reflectors(unsigned long long seed, int n, int nrefl, double kappa,
                                           double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n <= 0) return FLA_HIP_OK;
   if (nrefl < 0 || lda < n || !(kappa >= 1.0)) return FLA_HIP_EARG;
@@ -479,6 +488,7 @@ extern "C" int fla_hip_gen_spd_reflectors(unsigned long long seed, int n, int nr
 // v from seed + r + 1 and u from seed2 + r + 1 (fla_hip_gen_normal).
 extern "C" int fla_hip_gen_ge_reflectors(unsigned long long seed, unsigned long long seed2, int n,
                                          int nrefl, double kappa, double* A, int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (n <= 0) return FLA_HIP_OK;
   if (nrefl < 0 || lda < n || !(kappa >= 1.0)) return FLA_HIP_EARG;
@@ -524,6 +534,7 @@ __global__ void d2s_kernel(int m, int n, const double* __restrict__ A, long lda,
 // S := float(A) for an m x n matrix (round to nearest).  A and S must not
 // overlap.
 extern "C" int fla_hip_convert_d2s(int m, int n, const double* A, int lda, float* S, int lds) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || lds < (m > 1 ? m : 1)) return FLA_HIP_EARG;
   if (m == 0 || n == 0) return FLA_HIP_OK;
@@ -700,6 +711,7 @@ int norm_driver(int kind, int sym_mode, int m, int n, const double* A, long lda,
 }  // namespace
 
 extern "C" int fla_hip_dlange(int norm, int m, int n, const double* A, int lda, double* value) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   const int kind = norm_kind(norm);
   if (!value || kind == 0 || m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
@@ -710,6 +722,7 @@ extern "C" int fla_hip_dlange(int norm, int m, int n, const double* A, int lda, 
 
 extern "C" int fla_hip_dlansy(int norm, int uplo, int n, const double* A, int lda,
                               double* value) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   const int kind = norm_kind(norm);
   if (!value || kind == 0 || (uplo != kLower && uplo != kUpper) || n < 0 ||
@@ -722,6 +735,7 @@ extern "C" int fla_hip_dlansy(int norm, int uplo, int n, const double* A, int ld
 
 extern "C" int fla_hip_dlacpy(int uplo, int m, int n, const double* A, int lda, double* B,
                               int ldb) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || ldb < (m > 1 ? m : 1)) return FLA_HIP_EARG;
   if (m == 0 || n == 0) return FLA_HIP_OK;
@@ -734,6 +748,7 @@ extern "C" int fla_hip_dlacpy(int uplo, int m, int n, const double* A, int lda, 
 
 extern "C" int fla_hip_dlaset(int uplo, int m, int n, double alpha, double beta, double* A,
                               int lda) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
   if (m == 0 || n == 0) return FLA_HIP_OK;

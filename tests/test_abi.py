@@ -57,6 +57,7 @@ def test_python_binding_loads_without_gpu():
     L = fla.lib()
     L.FLA_Init()
     assert L.FLA_Initialized() == 1
+    assert L.fla_hip_get_stream() is None  # no caller's stream until one is set
     L.FLA_Finalize()
 
 
