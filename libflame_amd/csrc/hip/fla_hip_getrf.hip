@@ -873,6 +873,7 @@ static int lu_blocked2(FlaHipCtx& c, int m, int n, double* A, long lda, int* p, 
 }
 
 extern "C" int fla_hip_dgetrf(int m, int n, double* A, int lda, int* p, int* info) {
+  FLA_USER_STREAM_SCOPE;
   FLA_TRY(fla_ensure_ready());
   if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLA_HIP_EARG;
   *info = kSuccess;
