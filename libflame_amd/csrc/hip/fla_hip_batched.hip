@@ -200,7 +200,26 @@ __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, i
       P[r + c * MAXN] = A[(k + r) + (long)(k + c) * lda];
     }
     __syncthreads();
+    // One barrier per column: thread t owns rows t and t + 256 of the panel.
+    // In the phase of column j every thread updates its rows' columns
+    // j+1..kb-1 with the scaled multipliers formed on the fly (the same
+    // rounded products l = a * (1/d) the stored column holds), and keeps its
+    // rows' scaled column-j values in registers; they are stored in the next
+    // phase, when no thread reads column j any more.  Operations and order
+    // per element are those of the two-barrier sweep (dpotf2's order).
+    T pend[2] = {(T)0, (T)0};
+    T psd = (T)0;
+    int pj = -1;  // the column whose scaled values are pending
     for (int j = 0; j < kb; ++j) {
+      if (pj >= 0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int r = t + 256 * q;
+          if (r > pj && r < m) P[r + pj * MAXN] = pend[q];
+        }
+        if (t == 0) P[pj + pj * MAXN] = psd;
+        pj = -1;
+      }
       const T d = P[j + j * MAXN];
       if (!(d > (T)0)) {
         fail = k + j;
@@ -208,15 +227,27 @@ __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, i
       }
       const T sd = sqrt(d);
       const T inv = (T)1 / sd;
-      for (int r = j + 1 + t; r < m; r += 256) P[r + j * MAXN] *= inv;
-      __syncthreads();
-      if (t == 0) P[j + j * MAXN] = sd;
-      const int rr = m - j - 1, cc = kb - j - 1;
-      for (int e = t; e < rr * cc; e += 256) {
-        const int r = j + 1 + e % rr, c = j + 1 + e / rr;
-        if (r >= c) P[r + c * MAXN] -= P[r + j * MAXN] * P[c + j * MAXN];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int r = t + 256 * q;
+        if (r > j && r < m) {
+          const T lr = P[r + j * MAXN] * inv;
+          pend[q] = lr;
+          const int cmax = r < kb - 1 ? r : kb - 1;
+          for (int c = j + 1; c <= cmax; ++c) P[r + c * MAXN] -= lr * (P[c + j * MAXN] * inv);
+        }
       }
+      psd = sd;
+      pj = j;
       __syncthreads();
+    }
+    if (pj >= 0) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int r = t + 256 * q;
+        if (r > pj && r < m) P[r + pj * MAXN] = pend[q];
+      }
+      if (t == 0) P[pj + pj * MAXN] = psd;
     }
     __syncthreads();
     // write the factored panel back (lower part)
