@@ -19,7 +19,10 @@
 //     panel updated; then A12 := inv(L11) A12 and A22 -= L21 A12 on MFMA.
 // The forward substitution of the solve runs inside the factorization,
 // panel by panel, while L's panel is in LDS; the backward substitution reads
-// the factor once more from memory.
+// the factor once more from memory, with the right-hand sides staged in LDS.
+// Both solve 16 rows at a time: one wave sweeps the diagonal block with the
+// right-hand sides in registers (lanes exchange values by shuffles, no
+// barrier per column), then every thread updates the remaining rows.
 //
 // Element types: double (v_mfma_f64_16x16x4f64) and float
 // (v_mfma_f32_16x16x4f32).  Info per matrix: FLA_SUCCESS, or the 0-based
@@ -188,6 +191,77 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
   }
 }
 
+// One wave solves a kb x kb (kb <= 16) triangular block for up to 8
+// right-hand sides held in registers: lane r + 16 g holds row r of the
+// columns g and g + 4 (x[0], x[1]).  cf[j] is the coefficient of row r
+// against column j of the block and dg the diagonal of row r.  FWD sweeps
+// the columns in ascending order and updates the rows below each column;
+// otherwise the columns go in descending order and the rows above are
+// updated.  Per element the operations are the column sweep of dtrsm_
+// (src/flablas/f2c/dtrsm.c): x_j := x_j / d_j, then x_i -= c_ij x_j.
+template <typename T, bool FWD>
+__device__ __forceinline__ void wave_trsv16(T (&x)[2], const T (&cf)[16], T dg, int r, int g,
+                                            int kb) {
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int j = FWD ? s : 15 - s;
+    if (j < kb) {
+      if (r == j) {
+        x[0] = x[0] / dg;
+        x[1] = x[1] / dg;
+      }
+      const T y0 = __shfl(x[0], j + 16 * g, 64), y1 = __shfl(x[1], j + 16 * g, 64);
+      if (FWD ? r > j : r < j) {
+        x[0] -= cf[j] * y0;
+        x[1] -= cf[j] * y1;
+      }
+    }
+  }
+}
+
+// Forward substitution with the factored Cholesky panel in LDS (element
+// (i, j) at P[i + j*MAXN], m rows, kb columns): the panel's rows of B (B
+// points at the panel's first row) are solved against the diagonal block
+// and the rows below are updated, 16 columns at a time.  Each wave solves 8
+// right-hand sides per pass; the update subtracts one term at a time in
+// column order (dtrsm_'s order for B := inv(L) B).
+template <typename T>
+__device__ __forceinline__ void chol_panel_forward(const T* P, int kb, int m, T* B, long ldb, int nrhs, int t) {
+  const int w = t >> 6, lane = t & 63, r = lane & 15, g = lane >> 4;
+  for (int s0 = 0; s0 < kb; s0 += 16) {
+    const int sb = min(16, kb - s0), row = s0 + r;
+    if (8 * w < nrhs) {
+      T cf[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) cf[j] = (r < sb && j < sb) ? P[row + (s0 + j) * MAXN] : (T)0;
+      const T dg = r < sb ? P[row + row * MAXN] : (T)1;
+      for (int c0 = 8 * w; c0 < nrhs; c0 += 32) {
+        T x[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int c = c0 + g + 4 * q;
+          x[q] = (r < sb && c < nrhs) ? B[row + (long)c * ldb] : (T)0;
+        }
+        wave_trsv16<T, true>(x, cf, dg, r, g, sb);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int c = c0 + g + 4 * q;
+          if (r < sb && c < nrhs) B[row + (long)c * ldb] = x[q];
+        }
+      }
+    }
+    __syncthreads();
+    const int rows = m - s0 - sb;
+    for (int e = t; e < rows * nrhs; e += 256) {
+      const int i = s0 + sb + e % rows, c = e / rows;
+      T b = B[i + (long)c * ldb];
+      for (int j = 0; j < sb; ++j) b -= P[i + (s0 + j) * MAXN] * B[(s0 + j) + (long)c * ldb];
+      B[i + (long)c * ldb] = b;
+    }
+    __syncthreads();
+  }
+}
+
 // ---- Cholesky of one matrix (lower), fused forward substitution -----------
 template <typename T, int NB>
 __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, int t) {
@@ -257,23 +331,7 @@ __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, i
     }
     if (fail >= 0) return fail;
     // forward substitution with this panel: Y_k = L_kk^-1 B_k, B_rest -= L21 Y_k
-    if (nrhs > 0) {
-      for (int j = 0; j < kb; ++j) {
-        for (int c = t; c < nrhs; c += 256) B[(k + j) + (long)c * ldb] /= P[j + j * MAXN];
-        __syncthreads();
-        for (int e = t; e < (kb - j - 1) * nrhs; e += 256) {
-          const int r = j + 1 + e % (kb - j - 1), c = e / (kb - j - 1);
-          B[(k + r) + (long)c * ldb] -= P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
-        }
-        __syncthreads();
-      }
-      for (int e = t; e < (m - kb) * nrhs; e += 256) {
-        const int r = kb + e % (m - kb), c = e / (m - kb);
-        T s = (T)0;
-        for (int j = 0; j < kb; ++j) s += P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
-        B[(k + r) + (long)c * ldb] -= s;
-      }
-    }
+    if (nrhs > 0) chol_panel_forward<T>(P, kb, m, B + k, ldb, nrhs, t);
     // trailing update
     if (m > kb)
       tile_update<T, true>(m - kb, m - kb, kb, P + kb, P + kb, 1, MAXN,
@@ -283,37 +341,55 @@ __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, i
   return fail;
 }
 
-// Backward substitution X := L^-T Y (L lower in A), block by block from the
-// bottom, NB rows at a time.
-template <typename T, int NB>
-__device__ void chol_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* S, int t) {
-  const int nbk = (n + NB - 1) / NB;
-  for (int q = nbk - 1; q >= 0; --q) {
-    const int k = q * NB, kb = min(NB, n - k);
-    // B_k -= L[k+kb:n, k:k+kb]^T X[k+kb:n]
-    for (int e = t; e < kb * nrhs; e += 256) {
-      const int j = e % kb, c = e / kb;
-      T s = (T)0;
-      for (int i = k + kb; i < n; ++i) s += A[i + (long)(k + j) * lda] * B[i + (long)c * ldb];
-      S[j + c * NB] = s;
-    }
-    __syncthreads();
-    for (int e = t; e < kb * nrhs; e += 256) {
-      const int j = e % kb, c = e / kb;
-      B[(k + j) + (long)c * ldb] -= S[j + c * NB];
-    }
-    __syncthreads();
-    // X_k = L_kk^-T B_k (backward column sweep)
-    for (int j = kb - 1; j >= 0; --j) {
-      for (int c = t; c < nrhs; c += 256) B[(k + j) + (long)c * ldb] /= A[(k + j) + (long)(k + j) * lda];
-      __syncthreads();
-      for (int e = t; e < j * nrhs; e += 256) {
-        const int i = e % j, c = e / j;
-        B[(k + i) + (long)c * ldb] -= A[(k + j) + (long)(k + i) * lda] * B[(k + j) + (long)c * ldb];
+// Backward substitution X := L^-T Y (L lower in A) for up to 8 right-hand
+// sides, staged in LDS (X, ld MAXN).  From the bottom, 16 rows at a time:
+// one wave solves the diagonal block in registers, then every thread
+// updates the rows above with the block's solution.
+template <typename T>
+__device__ void chol_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* X, int t) {
+  const int w = t >> 6, lane = t & 63, r = lane & 15, g = lane >> 4;
+  for (int e = t; e < n * nrhs; e += 256) X[e % n + (e / n) * MAXN] = B[e % n + (long)(e / n) * ldb];
+  __syncthreads();
+  for (int k = ((n - 1) / 16) * 16; k >= 0; k -= 16) {
+    const int kb = min(16, n - k);
+    if (w == 0) {
+      // row r of the block of L^T: L[k + j, k + r]
+      T cf[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        cf[j] = (r < kb && j < kb) ? A[(k + j) + (long)(k + r) * lda] : (T)0;
+      const T dg = r < kb ? A[(k + r) + (long)(k + r) * lda] : (T)1;
+      T x[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c = g + 4 * q;
+        x[q] = (r < kb && c < nrhs) ? X[(k + r) + c * MAXN] : (T)0;
       }
-      __syncthreads();
+      wave_trsv16<T, false>(x, cf, dg, r, g, kb);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c = g + 4 * q;
+        if (r < kb && c < nrhs) X[(k + r) + c * MAXN] = x[q];
+      }
     }
+    __syncthreads();
+    for (int i = t; i < k; i += 256) {
+      T l[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) l[j] = j < kb ? A[(k + j) + (long)i * lda] : (T)0;
+#pragma unroll 1
+      for (int c = 0; c < nrhs; ++c) {
+        T v = X[i + c * MAXN];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+          if (j < kb) v -= l[j] * X[(k + j) + c * MAXN];
+        X[i + c * MAXN] = v;
+      }
+    }
+    __syncthreads();
   }
+  for (int e = t; e < n * nrhs; e += 256) B[e % n + (long)(e / n) * ldb] = X[e % n + (e / n) * MAXN];
+  __syncthreads();
 }
 
 // ---- LU with partial pivoting of one square matrix, fused forward solve ----
@@ -383,12 +459,14 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
         }
     __syncthreads();
     const int n2 = n - k - kb;
-    if (n2 > 0) {
-      // A12 := inv(L11) A12 (unit lower), one thread per column
-      for (int c = t; c < n2; c += 256) {
+    if (n2 + nrhs > 0) {
+      // A12 := inv(L11) A12 (unit lower), one thread per column; the panel's
+      // rows of the right-hand sides are solved alongside as further columns
+      for (int c = t; c < n2 + nrhs; c += 256) {
+        T* col = c < n2 ? A + k + (long)(k + kb + c) * lda : B + k + (long)(c - n2) * ldb;
         T x[NB];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) x[i] = (i < kb) ? A[(k + i) + (long)(k + kb + c) * lda] : (T)0;
+        for (int i = 0; i < NB; ++i) x[i] = (i < kb) ? col[i] : (T)0;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
           // x[j] is final here; keeps later columns' LDS reads from being
@@ -400,29 +478,25 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i)
-          if (i < kb) A[(k + i) + (long)(k + kb + c) * lda] = x[i];
+          if (i < kb) col[i] = x[i];
       }
       __threadfence_block();
       __syncthreads();
+    }
+    if (n2 > 0) {
       const T* U12 = A + k + (long)(k + kb) * lda;  // (kk, col) at U12[kk + col*lda]
       tile_update<T, false>(m - kb, n2, kb, P + kb, U12, lda, 1,
                             A + (k + kb) + (long)(k + kb) * lda, lda, t);
     }
-    // forward substitution of the solve (unit lower), rows already permuted
+    // forward substitution below the panel: b_i -= l_ij y_j, one term at a
+    // time in column order (dtrsm_'s order for B := inv(L) B)
     if (nrhs > 0) {
-      for (int j = 0; j < kb; ++j) {
-        __syncthreads();
-        for (int e = t; e < (kb - j - 1) * nrhs; e += 256) {
-          const int r = j + 1 + e % (kb - j - 1), c = e / (kb - j - 1);
-          B[(k + r) + (long)c * ldb] -= P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
-        }
-      }
-      __syncthreads();
-      for (int e = t; e < (m - kb) * nrhs; e += 256) {
-        const int r = kb + e % (m - kb), c = e / (m - kb);
-        T s = (T)0;
-        for (int j = 0; j < kb; ++j) s += P[r + j * MAXN] * B[(k + j) + (long)c * ldb];
-        B[(k + r) + (long)c * ldb] -= s;
+      const int rows = m - kb;
+      for (int e = t; e < rows * nrhs; e += 256) {
+        const int i = kb + e % rows, c = e / rows;
+        T b = B[(k + i) + (long)c * ldb];
+        for (int j = 0; j < kb; ++j) b -= P[i + j * MAXN] * B[(k + j) + (long)c * ldb];
+        B[(k + i) + (long)c * ldb] = b;
       }
     }
     __syncthreads();
@@ -430,34 +504,55 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
   return first_null;
 }
 
-// X := U^-1 Y (U upper in A), NB rows at a time from the bottom.
-template <typename T, int NB>
-__device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* S, int t) {
-  const int nbk = (n + NB - 1) / NB;
-  for (int q = nbk - 1; q >= 0; --q) {
-    const int k = q * NB, kb = min(NB, n - k);
-    for (int e = t; e < kb * nrhs; e += 256) {
-      const int j = e % kb, c = e / kb;
-      T s = (T)0;
-      for (int i = k + kb; i < n; ++i) s += A[(k + j) + (long)i * lda] * B[i + (long)c * ldb];
-      S[j + c * NB] = s;
-    }
-    __syncthreads();
-    for (int e = t; e < kb * nrhs; e += 256) {
-      const int j = e % kb, c = e / kb;
-      B[(k + j) + (long)c * ldb] -= S[j + c * NB];
-    }
-    __syncthreads();
-    for (int j = kb - 1; j >= 0; --j) {
-      for (int c = t; c < nrhs; c += 256) B[(k + j) + (long)c * ldb] /= A[(k + j) + (long)(k + j) * lda];
-      __syncthreads();
-      for (int e = t; e < j * nrhs; e += 256) {
-        const int i = e % j, c = e / j;
-        B[(k + i) + (long)c * ldb] -= A[(k + i) + (long)(k + j) * lda] * B[(k + j) + (long)c * ldb];
+// Backward substitution X := U^-1 Y (U upper in A) for up to 8 right-hand
+// sides, staged in LDS (X, ld MAXN).  From the bottom, 16 rows at a time:
+// one wave solves the diagonal block in registers, then every thread
+// updates the rows above, one term at a time in descending column order.
+// Per element this is dtrsm_'s column sweep for B := inv(U) B.
+template <typename T>
+__device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* X, int t) {
+  const int w = t >> 6, lane = t & 63, r = lane & 15, g = lane >> 4;
+  for (int e = t; e < n * nrhs; e += 256) X[e % n + (e / n) * MAXN] = B[e % n + (long)(e / n) * ldb];
+  __syncthreads();
+  for (int k = ((n - 1) / 16) * 16; k >= 0; k -= 16) {
+    const int kb = min(16, n - k);
+    if (w == 0) {
+      T cf[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        cf[j] = (r < kb && j < kb) ? A[(k + r) + (long)(k + j) * lda] : (T)0;
+      const T dg = r < kb ? A[(k + r) + (long)(k + r) * lda] : (T)1;
+      T x[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c = g + 4 * q;
+        x[q] = (r < kb && c < nrhs) ? X[(k + r) + c * MAXN] : (T)0;
       }
-      __syncthreads();
+      wave_trsv16<T, false>(x, cf, dg, r, g, kb);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c = g + 4 * q;
+        if (r < kb && c < nrhs) X[(k + r) + c * MAXN] = x[q];
+      }
     }
+    __syncthreads();
+    for (int i = t; i < k; i += 256) {
+      T u[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u[j] = j < kb ? A[i + (long)(k + j) * lda] : (T)0;
+#pragma unroll 1
+      for (int c = 0; c < nrhs; ++c) {
+        T v = X[i + c * MAXN];
+#pragma unroll
+        for (int j = 15; j >= 0; --j)
+          if (j < kb) v -= u[j] * X[(k + j) + c * MAXN];
+        X[i + c * MAXN] = v;
+      }
+    }
+    __syncthreads();
   }
+  for (int e = t; e < n * nrhs; e += 256) B[e % n + (long)(e / n) * ldb] = X[e % n + (e / n) * MAXN];
+  __syncthreads();
 }
 
 // Persistent driver: OP 0 Cholesky (lower), 1 LU with partial pivoting.
@@ -466,7 +561,6 @@ __global__ __launch_bounds__(256) void batched_kernel(const BatchItem* __restric
                                                       int count, int* counter, int* info,
                                                       int max_rhs) {
   __shared__ T P[MAXN * NB];
-  __shared__ T S[NB * 8];
   __shared__ AmaxSum red[OP == 1 ? 256 : 1];
   __shared__ int spiv[NB];
   __shared__ int cur;
@@ -488,14 +582,14 @@ __global__ __launch_bounds__(256) void batched_kernel(const BatchItem* __restric
       __syncthreads();
       if (r < 0 && nr > 0) {
         for (int c0 = 0; c0 < nr; c0 += 8)
-          chol_back<T, NB>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), S, t);
+          chol_back<T>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), P, t);
       }
     } else {
       r = lu_one<T, NB>(A, it.lda, it.n, it.ipiv, B, it.ldb, nr, P, red, spiv, t);
       __syncthreads();
       if (r < 0 && nr > 0) {
         for (int c0 = 0; c0 < nr; c0 += 8)
-          lu_back<T, NB>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), S, t);
+          lu_back<T>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), P, t);
       }
     }
     if (t == 0) info[it.index] = r < 0 ? kSuccess : r;

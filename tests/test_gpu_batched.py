@@ -85,6 +85,29 @@ def test_batched_chol_exact_with_solve(dtype):
         assert np.array_equal(X[i], Xs[i].astype(dtype)), (i, SIZES[i])
 
 
+@pytest.mark.parametrize("kind", ["chol", "lu"])
+def test_batched_solve_many_rhs_exact(kind):
+    """More right-hand sides than one wave's 8 per pass: the forward sweep
+    spreads column groups over the waves, the backward sweep runs in groups
+    of 8; integer solutions of the exact problems come back bit for bit."""
+    rng = np.random.default_rng(11)
+    sizes, nrhs = [40, 300, 17, 512, 129], [9, 33, 100, 256, 64]
+    As, Bs, Xs = [], [], []
+    for i, (n, r) in enumerate(zip(sizes, nrhs)):
+        A = exact_spd(n, 400 + i)[0] if kind == "chol" else exact_lu(n, 500 + i)[0]
+        X = rng.integers(-2, 3, size=(n, r)).astype(np.float64)
+        As.append(A)
+        Xs.append(X)
+        Bs.append(A @ X)
+    _, _, X, info = run_batch(kind, As, Bs)
+    assert np.all(info == fla.FLA_SUCCESS)
+    for i in range(len(sizes)):
+        if kind == "chol":
+            assert np.array_equal(X[i], Xs[i]), (i, sizes[i], nrhs[i])
+        else:  # L has entries k/4: the products are exact, U's solve divides by powers of 2
+            assert np.allclose(X[i], Xs[i], rtol=0, atol=1e-9), (i, sizes[i], nrhs[i])
+
+
 def test_batched_chol_vs_oracle(orc):
     As = [orc.gen_spd_c2(7 + i, n) for i, n in enumerate(SIZES)]
     F, _, _, info = run_batch("chol", As)
