@@ -94,53 +94,49 @@ __device__ __forceinline__ AmaxSum amax_combine(const AmaxSum& L, const AmaxSum&
   return o;
 }
 
-// Workgroup-wide idamax over rows [r0, m) of LDS column col (ld MAXN).
-// 256 threads, each a contiguous run of rows; result row (absolute) in *out.
-template <typename T>
-__device__ int block_idamax(const T* col, int r0, int m, AmaxSum* red, int t) {
-  const int len = m - r0;
-  const int per = (len + 255) / 256;
+// idamax_ summary of one element at position row.
+__device__ __forceinline__ AmaxSum amax_of(double a, int row) {
   AmaxSum s{0.0, -1, -1};
-  const int b = r0 + t * per, e = min(m, b + per);
-  for (int i = b; i < e; ++i) {
-    const double a = (double)fabs_t(col[i]);
-    if (a != a) {
-      s.nidx = i;
-      s.bidx = -1;
-    } else if (s.bidx < 0 || a > s.best) {
-      s.best = a;
-      s.bidx = i;
-    }
+  if (a != a)
+    s.nidx = row;
+  else {
+    s.best = a;
+    s.bidx = row;
   }
-  red[t] = s;
-  __syncthreads();
-  for (int st = 1; st < 256; st <<= 1) {
-    AmaxSum v;
-    const bool act = (t % (2 * st)) == 0 && t + st < 256;
-    if (act) v = amax_combine(red[t], red[t + st]);
-    __syncthreads();
-    if (act) red[t] = v;
-    __syncthreads();
+  return s;
+}
+
+// Combines the summaries of a wave's lanes in lane order (lane l holds a
+// run of rows that precedes lane l + 1's); every lane gets the result.
+__device__ __forceinline__ AmaxSum amax_wave(AmaxSum v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    AmaxSum o;
+    o.best = __shfl_xor(v.best, off, 64);
+    o.bidx = __shfl_xor(v.bidx, off, 64);
+    o.nidx = __shfl_xor(v.nidx, off, 64);
+    v = (lane & off) ? amax_combine(o, v) : amax_combine(v, o);
   }
-  const AmaxSum r = red[0];
-  __syncthreads();
-  return r.bidx >= 0 ? r.bidx : r.nidx;
+  return v;
 }
 
 // Trailing update C := C - Lr * Lc^T on MFMA, 32 x 32 tiles (lower tiles
-// only when LOWER) of the m2 x n2 block C (ld ldc).  Row operand: LDS panel
-// rows (element (row, kk) at Pr[row + kk*MAXN]); column operand element
-// (col, kk) at Pc[col*pcs + kk*pks] (the LDS panel for the Cholesky, U12 in
-// memory for the LU).
-template <typename T, bool LOWER>
+// only when LOWER) of the m2 x n2 block C (ld ldc), K = kb <= NB.  Row
+// operand: LDS panel rows (element (row, kk) at Pr[row + kk*MAXN]); column
+// operand element (col, kk) at Pc[col*pcs + kk*pks] (the LDS panel for the
+// Cholesky, U12 in memory for the LU: PC_GLOBAL).  Each wave walks its
+// tiles with the next tile's C (and, from memory, its column fragments)
+// loaded while the current tile's MFMAs run, so the global latency of one
+// tile overlaps the work of the previous one.
+template <typename T, bool LOWER, int NB, bool PC_GLOBAL>
 __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, long pcs, long pks,
                             T* C, long ldc, int t) {
   typedef typename Acc<T>::type AT;
+  constexpr int NK = NB / 4;  // k-steps of 4
   const int w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
   const int T1 = (m2 + 31) / 32, T2 = (n2 + 31) / 32;
   const int ntiles = LOWER ? T1 * (T1 + 1) / 2 : T1 * T2;
-  for (int tile = w; tile < ntiles; tile += 4) {
-    int ti, tj;
+  auto coords = [&](int tile, int& ti, int& tj) {
     if (LOWER) {
       tj = 0;
       int rem = tile;
@@ -153,6 +149,55 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
       ti = tile % T1;
       tj = tile / T1;
     }
+  };
+  auto in_c = [&](int i, int j) { return i < m2 && j < n2 && (!LOWER || i >= j); };
+  // loads C and (PC_GLOBAL) the column fragments of one tile
+  auto load = [&](int ti, int tj, T (&cv)[2][2][4], T (&pf)[NK][2]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 32 + a * 16 + li, j = tj * 32 + b * 16 + Acc<T>::dj(lk, r);
+          cv[a][b][r] = in_c(i, j) ? C[i + (long)j * ldc] : (T)0;
+        }
+    if (PC_GLOBAL) {
+#pragma unroll
+      for (int q = 0; q < NK; ++q) {
+        const int kk = 4 * q + lk;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const int col = tj * 32 + b * 16 + li;
+          pf[q][b] = (col < n2 && kk < kb) ? Pc[(long)col * pcs + (long)kk * pks] : (T)0;
+        }
+      }
+    }
+  };
+  int tile = w;
+  if (tile >= ntiles) return;
+  int ti, tj;
+  coords(tile, ti, tj);
+  T cn[2][2][4], pn[NK][2];
+  load(ti, tj, cn, pn);
+  for (; tile < ntiles; tile += 4) {
+    T cv[2][2][4], pf[NK][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[a][b][r] = cn[a][b][r];
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+      pf[q][0] = pn[q][0];
+      pf[q][1] = pn[q][1];
+    }
+    const int ci = ti, cj = tj;
+    if (tile + 4 < ntiles) {
+      coords(tile + 4, ti, tj);
+      load(ti, tj, cn, pn);
+    }
     AT acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -160,23 +205,27 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a][b][r] = (T)0;
-    for (int ks = 0; ks < kb; ks += 4) {
-      const int kk = ks + lk;
-      T rf[2], cf[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int row = ti * 32 + a * 16 + li;
-        rf[a] = (row < m2 && kk < kb) ? Pr[row + kk * MAXN] : (T)0;
+    for (int q = 0; q < NK; ++q) {
+      if (4 * q < kb) {
+        const int kk = 4 * q + lk;
+        T rf[2], cf[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const int row = ci * 32 + a * 16 + li;
+          rf[a] = (row < m2 && kk < kb) ? Pr[row + kk * MAXN] : (T)0;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const int col = cj * 32 + b * 16 + li;
+          cf[b] = PC_GLOBAL ? pf[q][b]
+                            : ((col < n2 && kk < kb) ? Pc[(long)col * pcs + (long)kk * pks] : (T)0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) acc[a][b] = mfma4(cf[b], rf[a], acc[a][b]);
       }
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int col = tj * 32 + b * 16 + li;
-        cf[b] = (col < n2 && kk < kb) ? Pc[(long)col * pcs + (long)kk * pks] : (T)0;
-      }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = mfma4(cf[b], rf[a], acc[a][b]);
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -184,9 +233,9 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = ti * 32 + a * 16 + li;
-          const int j = tj * 32 + b * 16 + Acc<T>::dj(lk, r);
-          if (i < m2 && j < n2 && (!LOWER || i >= j)) C[i + (long)j * ldc] -= acc[a][b][r];
+          const int i = ci * 32 + a * 16 + li;
+          const int j = cj * 32 + b * 16 + Acc<T>::dj(lk, r);
+          if (in_c(i, j)) C[i + (long)j * ldc] = cv[a][b][r] - acc[a][b][r];
         }
   }
 }
@@ -334,7 +383,7 @@ __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, i
     if (nrhs > 0) chol_panel_forward<T>(P, kb, m, B + k, ldb, nrhs, t);
     // trailing update
     if (m > kb)
-      tile_update<T, true>(m - kb, m - kb, kb, P + kb, P + kb, 1, MAXN,
+      tile_update<T, true, NB, false>(m - kb, m - kb, kb, P + kb, P + kb, 1, MAXN,
                            A + (k + kb) + (long)(k + kb) * lda, lda, t);
     __syncthreads();
   }
@@ -404,10 +453,32 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
       P[r + c * MAXN] = A[(k + r) + (long)(k + c) * lda];
     }
     __syncthreads();
+    // Thread t owns rows t and t + 256 of the panel.  Per column: the
+    // owners' candidates are combined in row order (shuffles within a
+    // wave, the four waves' two partial results through LDS: one barrier),
+    // the pivot row is swapped in (second barrier), and each thread scales
+    // and updates its own rows with the multipliers formed on the spot
+    // (the same rounded products l = a * (1/piv) the stored column holds).
+    // The next column's search reads only the thread's own rows, so no
+    // third barrier is needed.
+    const int lane = t & 63, w = t >> 6;
     for (int j = 0; j < kb; ++j) {
-      const int pr = block_idamax<T>(P + j * MAXN, j, m, red, t);
+      const AmaxSum none{0.0, -1, -1};
+      AmaxSum s1 = none, s2 = none;
+      if (t >= j && t < m) s1 = amax_of((double)fabs_t(P[t + j * MAXN]), t);
+      if (t + 256 >= j && t + 256 < m) s2 = amax_of((double)fabs_t(P[t + 256 + j * MAXN]), t + 256);
+      s1 = amax_wave(s1, lane);
+      s2 = amax_wave(s2, lane);
+      if (lane == 0) {
+        red[w] = s1;
+        red[4 + w] = s2;
+      }
+      __syncthreads();
+      AmaxSum tot = red[0];
+#pragma unroll
+      for (int q = 1; q < 8; ++q) tot = amax_combine(tot, red[q]);
+      const int pr = tot.bidx >= 0 ? tot.bidx : tot.nidx;
       if (t == 0) spiv[j] = pr - j;  // relative pivot
-      // swap rows j and pr inside the panel
       if (pr != j)
         for (int c = t; c < kb; c += 256) {
           const T a = P[j + c * MAXN], b = P[pr + c * MAXN];
@@ -416,47 +487,65 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
         }
       __syncthreads();
       const T piv = P[j + j * MAXN];
-      if (piv != (T)0) {
-        const T inv = (T)1 / piv;
-        for (int r = j + 1 + t; r < m; r += 256) P[r + j * MAXN] *= inv;
-      } else if (first_null < 0) {
-        first_null = k + j;
+      const bool scale = piv != (T)0;
+      const T inv = scale ? (T)1 / piv : (T)1;
+      if (!scale && first_null < 0) first_null = k + j;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int r = t + 256 * q;
+        if (r > j && r < m) {
+          T l = P[r + j * MAXN];
+          if (scale) {
+            l *= inv;
+            P[r + j * MAXN] = l;
+          }
+          for (int c = j + 1; c < kb; ++c) P[r + c * MAXN] -= l * P[j + c * MAXN];
+        }
       }
-      __syncthreads();
-      const int rr = m - j - 1, cc = kb - j - 1;
-      for (int e = t; e < rr * cc; e += 256) {
-        const int r = j + 1 + e % rr, c = j + 1 + e / rr;
-        P[r + c * MAXN] -= P[r + j * MAXN] * P[j + c * MAXN];
-      }
-      __syncthreads();
     }
-    // panel back to A; pivots out; swaps applied to the columns left and right
+    __syncthreads();
+    // panel back to A; pivots out
     for (int e = t; e < m * kb; e += 256) {
       const int r = e % m, c = e / m;
       A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
     }
     for (int j = t; j < kb; j += 256) ipiv[k + j] = spiv[j];
-    for (int c = t; c < n; c += 256) {
-      if (c >= k && c < k + kb) continue;
-      for (int j = 0; j < kb; ++j) {
-        const int pr = k + j + spiv[j];
-        if (pr != k + j) {
-          const T a = A[(k + j) + (long)c * lda];
-          A[(k + j) + (long)c * lda] = A[pr + (long)c * lda];
-          A[pr + (long)c * lda] = a;
-        }
-      }
-    }
-    if (nrhs > 0)
-      for (int c = t; c < nrhs; c += 256)
-        for (int j = 0; j < kb; ++j) {
-          const int pr = k + j + spiv[j];
-          if (pr != k + j) {
-            const T a = B[(k + j) + (long)c * ldb];
-            B[(k + j) + (long)c * ldb] = B[pr + (long)c * ldb];
-            B[pr + (long)c * ldb] = a;
+    // The interchanges on the columns left and right of the panel and on the
+    // right-hand sides: each thread keeps the panel's kb rows of one column
+    // in registers and applies the kb swaps in order (a pivot row inside the
+    // panel is a register exchange, one below it a load and a store).
+    for (int c = t; c < n - kb + nrhs; c += 256) {
+      T* col = c < k ? A + (long)c * lda
+                     : (c < n - kb ? A + (long)(c + kb) * lda : B + (long)(c - (n - kb)) * ldb);
+      T x[NB];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) x[i] = i < kb ? col[k + i] : (T)0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < kb) {
+          const int i2 = j + spiv[j];
+          if (i2 != j) {
+            if (i2 < kb) {
+#pragma unroll
+              for (int i = j + 1; i < NB; ++i)
+                if (i == i2) {
+                  const T tmp = x[j];
+                  x[j] = x[i];
+                  x[i] = tmp;
+                }
+            } else {
+              T* p = col + k + i2;
+              const T v = *p;
+              *p = x[j];
+              x[j] = v;
+            }
           }
         }
+      }
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        if (i < kb) col[k + i] = x[i];
+    }
     __syncthreads();
     const int n2 = n - k - kb;
     if (n2 + nrhs > 0) {
@@ -485,7 +574,7 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
     }
     if (n2 > 0) {
       const T* U12 = A + k + (long)(k + kb) * lda;  // (kk, col) at U12[kk + col*lda]
-      tile_update<T, false>(m - kb, n2, kb, P + kb, U12, lda, 1,
+      tile_update<T, false, NB, true>(m - kb, n2, kb, P + kb, U12, lda, 1,
                             A + (k + kb) + (long)(k + kb) * lda, lda, t);
     }
     // forward substitution below the panel: b_i -= l_ij y_j, one term at a
@@ -561,7 +650,7 @@ __global__ __launch_bounds__(256) void batched_kernel(const BatchItem* __restric
                                                       int count, int* counter, int* info,
                                                       int max_rhs) {
   __shared__ T P[MAXN * NB];
-  __shared__ AmaxSum red[OP == 1 ? 256 : 1];
+  __shared__ AmaxSum red[8];
   __shared__ int spiv[NB];
   __shared__ int cur;
   const int t = threadIdx.x;
