@@ -300,7 +300,7 @@ int fla_hip_dgesv_batched(int count, const int* n, double* const* A, const int* 
 int fla_hip_batched_tune(int kind, int count, const int* n, const int* nrhs, int* best_nb,
                          int* best_wg_per_cu, double* best_ms);
 /* Forces one configuration for every batch of that kind (nb: 16 or 32 for
-   kinds 0 and 2, 16, 32 or 64 for kind 1; 1 <= wg_per_cu <= 4). */
+   kinds 0 and 1, 16 for kind 2; 1 <= wg_per_cu <= 4). */
 int fla_hip_batched_set_config(int kind, int nb, int wg_per_cu);
 int fla_hip_batched_set_graph(int on);
 

@@ -307,6 +307,56 @@ __device__ __forceinline__ void wave_trsv16(T (&x)[2], const T (&cf)[16], T dg, 
   }
 }
 
+// Copies the m x kb panel at Ak (ld lda) into P (ld MAXN).  Thread t owns
+// rows t and t + 256; all its loads are issued before the LDS stores.
+template <typename T, int NB>
+__device__ __forceinline__ void load_panel(const T* Ak, long lda, int kb, int m, T* P, int t) {
+  T v[2][NB];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int r = t + 256 * q;
+      v[q][c] = (r < m && c < kb) ? Ak[r + (long)c * lda] : (T)0;
+    }
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int r = t + 256 * q;
+      if (r < m && c < kb) P[r + c * MAXN] = v[q][c];
+    }
+}
+
+// Rows [i0, m) of B (relative to the panel; columns cg .. cg + ncg - 1,
+// ncg <= 32): b_ic -= l_ij y_jc for j < sb, one term at a time in column
+// order, with l_ij = P[i, p0 + j] and y_jc = Ys[j + (c - cg)*NY].  Each
+// thread keeps its row's sb multipliers in registers and works on 8
+// right-hand sides at a time.
+template <typename T, int NY>
+__device__ __forceinline__ void rows_update(const T* P, int p0, int sb, int i0, int m, T* B,
+                                            long ldb, int cg, int ncg, const T* Ys, int t) {
+  for (int i = i0 + t; i < m; i += 256) {
+    T l[NY];
+#pragma unroll
+    for (int j = 0; j < NY; ++j) l[j] = j < sb ? P[i + (p0 + j) * MAXN] : (T)0;
+    for (int c8 = 0; c8 < ncg; c8 += 8) {
+      T b[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) b[q] = c8 + q < ncg ? B[i + (long)(cg + c8 + q) * ldb] : (T)0;
+#pragma unroll
+      for (int j = 0; j < NY; ++j)
+        if (j < sb) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) b[q] -= l[j] * Ys[j + (c8 + q) * NY];
+        }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (c8 + q < ncg) B[i + (long)(cg + c8 + q) * ldb] = b[q];
+    }
+  }
+}
+
 // Forward substitution with the factored Cholesky panel in LDS (element
 // (i, j) at P[i + j*MAXN], m rows, kb columns): the panel's rows of B (B
 // points at the panel's first row) are solved against the diagonal block
@@ -314,53 +364,49 @@ __device__ __forceinline__ void wave_trsv16(T (&x)[2], const T (&cf)[16], T dg, 
 // right-hand sides per pass; the update subtracts one term at a time in
 // column order (dtrsm_'s order for B := inv(L) B).
 template <typename T>
-__device__ __forceinline__ void chol_panel_forward(const T* P, int kb, int m, T* B, long ldb, int nrhs, int t) {
+__device__ __forceinline__ void chol_panel_forward(const T* P, int kb, int m, T* B, long ldb,
+                                                   int nrhs, T* Ys, int t) {
   const int w = t >> 6, lane = t & 63, r = lane & 15, g = lane >> 4;
   for (int s0 = 0; s0 < kb; s0 += 16) {
     const int sb = min(16, kb - s0), row = s0 + r;
-    if (8 * w < nrhs) {
-      T cf[16];
+    for (int cg = 0; cg < nrhs; cg += 32) {
+      const int ncg = min(32, nrhs - cg);
+      if (8 * w < ncg) {
+        T cf[16];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) cf[j] = (r < sb && j < sb) ? P[row + (s0 + j) * MAXN] : (T)0;
-      const T dg = r < sb ? P[row + row * MAXN] : (T)1;
-      for (int c0 = 8 * w; c0 < nrhs; c0 += 32) {
+        for (int j = 0; j < 16; ++j) cf[j] = (r < sb && j < sb) ? P[row + (s0 + j) * MAXN] : (T)0;
+        const T dg = r < sb ? P[row + row * MAXN] : (T)1;
         T x[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const int c = c0 + g + 4 * q;
-          x[q] = (r < sb && c < nrhs) ? B[row + (long)c * ldb] : (T)0;
+          const int cc = 8 * w + g + 4 * q;
+          x[q] = (r < sb && cc < ncg) ? B[row + (long)(cg + cc) * ldb] : (T)0;
         }
         wave_trsv16<T, true>(x, cf, dg, r, g, sb);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const int c = c0 + g + 4 * q;
-          if (r < sb && c < nrhs) B[row + (long)c * ldb] = x[q];
+          const int cc = 8 * w + g + 4 * q;
+          if (r < sb && cc < ncg) {
+            B[row + (long)(cg + cc) * ldb] = x[q];
+            Ys[r + cc * 16] = x[q];
+          }
         }
       }
+      __syncthreads();
+      rows_update<T, 16>(P, s0, sb, s0 + sb, m, B, ldb, cg, ncg, Ys, t);
+      __syncthreads();
     }
-    __syncthreads();
-    const int rows = m - s0 - sb;
-    for (int e = t; e < rows * nrhs; e += 256) {
-      const int i = s0 + sb + e % rows, c = e / rows;
-      T b = B[i + (long)c * ldb];
-      for (int j = 0; j < sb; ++j) b -= P[i + (s0 + j) * MAXN] * B[(s0 + j) + (long)c * ldb];
-      B[i + (long)c * ldb] = b;
-    }
-    __syncthreads();
   }
 }
 
 // ---- Cholesky of one matrix (lower), fused forward substitution -----------
 template <typename T, int NB>
-__device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, int t) {
+__device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, T* Ys, int t) {
   int fail = -1;
   for (int k = 0; k < n; k += NB) {
     const int kb = min(NB, n - k), m = n - k;
     // panel A[k:n, k:k+kb] -> P (ld MAXN)
-    for (int e = t; e < m * kb; e += 256) {
-      const int r = e % m, c = e / m;
-      P[r + c * MAXN] = A[(k + r) + (long)(k + c) * lda];
-    }
+    load_panel<T, NB>(A + k + (long)k * lda, lda, kb, m, P, t);
     __syncthreads();
     // One barrier per column: thread t owns rows t and t + 256 of the panel.
     // In the phase of column j every thread updates its rows' columns
@@ -413,13 +459,11 @@ __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, i
     }
     __syncthreads();
     // write the factored panel back (lower part)
-    for (int e = t; e < m * kb; e += 256) {
-      const int r = e % m, c = e / m;
-      if (r >= c) A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
-    }
+    for (int c = 0; c < kb; ++c)
+      for (int r = t + c; r < m; r += 256) A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
     if (fail >= 0) return fail;
     // forward substitution with this panel: Y_k = L_kk^-1 B_k, B_rest -= L21 Y_k
-    if (nrhs > 0) chol_panel_forward<T>(P, kb, m, B + k, ldb, nrhs, t);
+    if (nrhs > 0) chol_panel_forward<T>(P, kb, m, B + k, ldb, nrhs, Ys, t);
     // trailing update
     if (m > kb)
       tile_update<T, true, NB, false>(m - kb, m - kb, kb, P + kb, P + kb, 1, MAXN,
@@ -483,14 +527,13 @@ __device__ void chol_back(const T* A, long lda, int n, T* B, long ldb, int nrhs,
 // ---- LU with partial pivoting of one square matrix, fused forward solve ----
 template <typename T, int NB>
 __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs, T* P,
-                      AmaxSum* red, int* spiv, int t) {
+                      AmaxSum* red, int* spiv, T* Ys, int t) {
+  __shared__ int prow[2 * NB], psrc[2 * NB], pcnt;
+  __shared__ int allpiv[MAXN];
   int first_null = -1;
   for (int k = 0; k < n; k += NB) {
     const int kb = min(NB, n - k), m = n - k;
-    for (int e = t; e < m * kb; e += 256) {
-      const int r = e % m, c = e / m;
-      P[r + c * MAXN] = A[(k + r) + (long)(k + c) * lda];
-    }
+    load_panel<T, NB>(A + k + (long)k * lda, lda, kb, m, P, t);
     __syncthreads();
     // Thread t owns rows t and t + 256 of the panel.  Per column: the
     // owners' candidates are combined in row order (shuffles within a
@@ -517,7 +560,10 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
 #pragma unroll
       for (int q = 1; q < 8; ++q) tot = amax_combine(tot, red[q]);
       const int pr = tot.bidx >= 0 ? tot.bidx : tot.nidx;
-      if (t == 0) spiv[j] = pr - j;  // relative pivot
+      if (t == 0) {
+        spiv[j] = pr - j;  // relative pivot
+        allpiv[k + j] = pr - j;
+      }
       if (pr != j)
         for (int c = t; c < kb; c += 256) {
           const T a = P[j + c * MAXN], b = P[pr + c * MAXN];
@@ -544,46 +590,58 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
     }
     __syncthreads();
     // panel back to A; pivots out
-    for (int e = t; e < m * kb; e += 256) {
-      const int r = e % m, c = e / m;
-      A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
-    }
+    for (int c = 0; c < kb; ++c)
+      for (int r = t; r < m; r += 256) A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
     for (int j = t; j < kb; j += 256) ipiv[k + j] = spiv[j];
-    // The interchanges on the columns left and right of the panel and on the
-    // right-hand sides: each thread keeps the panel's kb rows of one column
-    // in registers and applies the kb swaps in order (a pivot row inside the
-    // panel is a register exchange, one below it a load and a store).
-    for (int c = t; c < n - kb + nrhs; c += 256) {
-      T* col = c < k ? A + (long)c * lda
-                     : (c < n - kb ? A + (long)(c + kb) * lda : B + (long)(c - (n - kb)) * ldb);
-      T x[NB];
-#pragma unroll
-      for (int i = 0; i < NB; ++i) x[i] = i < kb ? col[k + i] : (T)0;
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        if (j < kb) {
-          const int i2 = j + spiv[j];
-          if (i2 != j) {
-            if (i2 < kb) {
-#pragma unroll
-              for (int i = j + 1; i < NB; ++i)
-                if (i == i2) {
-                  const T tmp = x[j];
-                  x[j] = x[i];
-                  x[i] = tmp;
-                }
-            } else {
-              T* p = col + k + i2;
-              const T v = *p;
-              *p = x[j];
-              x[j] = v;
+    // The kb interchanges compose into one permutation of at most 2 kb rows:
+    // slot s moves row psrc[s] to row prow[s] (relative to k).  Wave 0
+    // builds it with one lane per slot, applying the swaps in order.  Every
+    // column right of the panel and every right-hand side then reads all its
+    // moved rows before it writes any (no read-modify-write chain per swap).
+    // The columns left of the panel get the interchanges of all later
+    // panels at once, after the last panel.
+    if (w == 0) {
+      int row = lane < kb ? lane : -1, src = row, nt = kb;
+      for (int j = 0; j < kb; ++j) {
+        const int i2 = j + spiv[j];
+        if (i2 != j) {
+          const unsigned long long hit = __ballot(row == i2);
+          int s2;
+          if (hit) {
+            s2 = __ffsll((unsigned long long)hit) - 1;
+          } else {
+            s2 = nt;
+            if (lane == nt) {
+              row = i2;
+              src = i2;
             }
+            ++nt;
           }
+          const int sj = __shfl(src, j, 64), ss = __shfl(src, s2, 64);
+          if (lane == j) src = ss;
+          if (lane == s2) src = sj;
         }
       }
+      if (lane < nt) {
+        prow[lane] = row;
+        psrc[lane] = src;
+      }
+      if (lane == 0) pcnt = nt;
+    }
+    __syncthreads();
+    {
+      const int nt = pcnt;
+      const int nright = n - k - kb;
+      for (int c = t; c < nright + nrhs; c += 256) {
+        T* col = c < nright ? A + (long)(k + kb + c) * lda : B + (long)(c - nright) * ldb;
+        T v[2 * NB];
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
-        if (i < kb) col[k + i] = x[i];
+        for (int q = 0; q < 2 * NB; ++q)
+          if (q < nt && psrc[q] != prow[q]) v[q] = col[k + psrc[q]];
+#pragma unroll
+        for (int q = 0; q < 2 * NB; ++q)
+          if (q < nt && psrc[q] != prow[q]) col[k + prow[q]] = v[q];
+      }
     }
     __syncthreads();
     const int n2 = n - k - kb;
@@ -617,14 +675,52 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
                             A + (k + kb) + (long)(k + kb) * lda, lda, t);
     }
     // forward substitution below the panel: b_i -= l_ij y_j, one term at a
-    // time in column order (dtrsm_'s order for B := inv(L) B)
-    if (nrhs > 0) {
-      const int rows = m - kb;
-      for (int e = t; e < rows * nrhs; e += 256) {
-        const int i = kb + e % rows, c = e / rows;
-        T b = B[(k + i) + (long)c * ldb];
-        for (int j = 0; j < kb; ++j) b -= P[i + j * MAXN] * B[(k + j) + (long)c * ldb];
-        B[(k + i) + (long)c * ldb] = b;
+    // time in column order (dtrsm_'s order for B := inv(L) B), with the
+    // panel's solved rows of B staged in LDS
+    for (int cg = 0; cg < nrhs; cg += 32) {
+      const int ncg = min(32, nrhs - cg);
+      for (int cc = t / NB; cc < ncg; cc += 256 / NB) {
+        const int j = t % NB;
+        if (j < kb) Ys[j + cc * NB] = B[(k + j) + (long)(cg + cc) * ldb];
+      }
+      __syncthreads();
+      rows_update<T, NB>(P, 0, kb, kb, m, B + k, ldb, cg, ncg, Ys, t);
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  // The interchanges of later panels on the columns of earlier ones, once
+  // per matrix.  From the last panel back to the first, cont[q] is the row
+  // whose content ends up in row t + 256 q after the swaps of the panels
+  // after the current one; those swaps touch only rows below the current
+  // panel, so its columns gather rows there (all reads before any write).
+  {
+    int cont[2] = {t, t + 256};
+    for (int k0 = ((n - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+      const int kb = min(NB, n - k0);
+      T v[2][NB];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int r = t + 256 * q;
+        const bool mv = r >= k0 + kb && r < n && cont[q] != r;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          v[q][c] = (mv && c < kb) ? A[cont[q] + (long)(k0 + c) * lda] : (T)0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int r = t + 256 * q;
+        const bool mv = r >= k0 + kb && r < n && cont[q] != r;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (mv && c < kb) A[r + (long)(k0 + c) * lda] = v[q][c];
+      }
+      // cont := the swaps of this panel, last to first, applied to cont
+      for (int j = kb - 1; j >= 0; --j) {
+        const int a = k0 + j, b = a + allpiv[a];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) cont[q] = cont[q] == a ? b : (cont[q] == b ? a : cont[q]);
       }
     }
     __syncthreads();
@@ -688,12 +784,13 @@ __device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T
 // register budget is held to that occupancy.
 template <typename T, int OP, int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    (160 * 1024) / (MAXN * NB * sizeof(T) + 1024) >= 4
+    (160 * 1024) / ((MAXN + 32) * NB * sizeof(T) + 1024) >= 4
         ? 4
-        : ((160 * 1024) / (MAXN * NB * sizeof(T) + 1024) >= 2 ? 2 : 1)))) void batched_kernel(const BatchItem* __restrict__ items,
+        : ((160 * 1024) / ((MAXN + 32) * NB * sizeof(T) + 1024) >= 2 ? 2 : 1)))) void batched_kernel(const BatchItem* __restrict__ items,
                                                       int count, int* counter, int* info,
                                                       int max_rhs) {
   __shared__ T P[MAXN * NB];
+  __shared__ T Ys[NB * 32];  // solved rows of the right-hand sides, 32 columns
   __shared__ AmaxSum red[8];
   __shared__ int spiv[NB];
   __shared__ int cur;
@@ -711,14 +808,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     const int nr = it.B ? it.nrhs : 0;
     int r;
     if (OP == 0) {
-      r = chol_one<T, NB>(A, it.lda, it.n, B, it.ldb, nr, P, t);
+      r = chol_one<T, NB>(A, it.lda, it.n, B, it.ldb, nr, P, Ys, t);
       __syncthreads();
       if (r < 0 && nr > 0) {
         for (int c0 = 0; c0 < nr; c0 += 8)
           chol_back<T>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), P, t);
       }
     } else {
-      r = lu_one<T, NB>(A, it.lda, it.n, it.ipiv, B, it.ldb, nr, P, red, spiv, t);
+      r = lu_one<T, NB>(A, it.lda, it.n, it.ipiv, B, it.ldb, nr, P, red, spiv, Ys, t);
       __syncthreads();
       if (r < 0 && nr > 0) {
         for (int c0 = 0; c0 < nr; c0 += 8)
@@ -814,21 +911,21 @@ int launch_kind(hipStream_t s, int kind, const BatchConfig& cfg, int count, int*
   }
   if (kind == kSpotrf) {
     if (cfg.nb == 16) return launch_batched<float, 0, 16>(s, grid, g_items_dev, count, g_counter, info);
-    if (cfg.nb == 64) return launch_batched<float, 0, 64>(s, grid, g_items_dev, count, g_counter, info);
     return launch_batched<float, 0, 32>(s, grid, g_items_dev, count, g_counter, info);
   }
-  if (cfg.nb == 16) return launch_batched<double, 1, 16>(s, grid, g_items_dev, count, g_counter, info);
-  return launch_batched<double, 1, 32>(s, grid, g_items_dev, count, g_counter, info);
+  return launch_batched<double, 1, 16>(s, grid, g_items_dev, count, g_counter, info);
 }
 
+// Panel widths: 16 or 32 for the Cholesky kinds, 16 for the LU (wider LU
+// panels spill registers and measured slower).
 bool valid_nb(int kind, int nb) {
-  if (kind == kSpotrf) return nb == 16 || nb == 32 || nb == 64;
+  if (kind == kDgetrf) return nb == 16;
   return nb == 16 || nb == 32;
 }
 
 // Workgroups per CU that the panel's LDS allows (at most 4).
 int max_wg_per_cu(int kind, int nb) {
-  const int bytes = MAXN * nb * (kind == kSpotrf ? 4 : 8) + 1024;
+  const int bytes = (MAXN + 32) * nb * (kind == kSpotrf ? 4 : 8) + 1024;
   return std::max(1, std::min(4, (160 * 1024) / bytes));
 }
 
@@ -1044,7 +1141,7 @@ extern "C" int fla_hip_batched_tune(int kind, int count, const int* n, const int
     Pp[i] = ipiv_base + (size_t)i * MAXN;
   }
   std::vector<BatchConfig> cands;
-  for (int nb : (f32 ? std::vector<int>{16, 32, 64} : std::vector<int>{16, 32}))
+  for (int nb : (kind == kDgetrf ? std::vector<int>{16} : std::vector<int>{16, 32}))
     for (int wg = 1; wg <= max_wg_per_cu(kind, nb); wg *= 2) {
       BatchConfig c;
       c.nb = nb;
