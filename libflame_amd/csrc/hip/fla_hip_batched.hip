@@ -120,6 +120,35 @@ __device__ __forceinline__ AmaxSum amax_wave(AmaxSum v, int lane) {
   return v;
 }
 
+// Maximum of v over the wave by DPP moves (quad exchanges, row mirrors,
+// row broadcasts), returned uniform.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, ROWMASK,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, ROWMASK,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+  v = fmax(v, dpp_f64<0xB1, 0xF>(v));   // quad_perm [1, 0, 3, 2]
+  v = fmax(v, dpp_f64<0x4E, 0xF>(v));   // quad_perm [2, 3, 0, 1]
+  v = fmax(v, dpp_f64<0x141, 0xF>(v));  // row_half_mirror
+  v = fmax(v, dpp_f64<0x140, 0xF>(v));  // row_mirror
+  v = fmax(v, dpp_f64<0x142, 0xA>(v));  // row_bcast:15 into rows 1 and 3
+  v = fmax(v, dpp_f64<0x143, 0xC>(v));  // row_bcast:31 into rows 2 and 3
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// One wave's pivot candidates: its largest |a| and the first of its rows
+// holding it among rows 0..255 (i1) and 256..511 (i2); nan if any of its
+// candidates is a NaN.
+struct PivRed {
+  double best;
+  int i1, i2, nan, pad;
+};
+
 // Trailing update C := C - Lr * Lc^T on MFMA, 32 x 32 tiles (lower tiles
 // only when LOWER) of the m2 x n2 block C (ld ldc), K = kb <= NB.  Row
 // operand: LDS panel rows (element (row, kk) at Pr[row + kk*MAXN]); column
@@ -530,6 +559,7 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
                       AmaxSum* red, int* spiv, T* Ys, int t) {
   __shared__ int prow[2 * NB], psrc[2 * NB], pcnt;
   __shared__ int allpiv[MAXN];
+  __shared__ PivRed fred[4];
   int first_null = -1;
   for (int k = 0; k < n; k += NB) {
     const int kb = min(NB, n - k), m = n - k;
@@ -545,21 +575,49 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
     // third barrier is needed.
     const int lane = t & 63, w = t >> 6;
     for (int j = 0; j < kb; ++j) {
-      const AmaxSum none{0.0, -1, -1};
-      AmaxSum s1 = none, s2 = none;
-      if (t >= j && t < m) s1 = amax_of((double)fabs_t(P[t + j * MAXN]), t);
-      if (t + 256 >= j && t + 256 < m) s2 = amax_of((double)fabs_t(P[t + 256 + j * MAXN]), t + 256);
-      s1 = amax_wave(s1, lane);
-      s2 = amax_wave(s2, lane);
+      // candidates of the thread's rows: |a|, or -1 outside rows [j, m)
+      const double a1 = (t >= j && t < m) ? (double)fabs_t(P[t + j * MAXN]) : -1.0;
+      const double a2 =
+          (t + 256 >= j && t + 256 < m) ? (double)fabs_t(P[t + 256 + j * MAXN]) : -1.0;
+      const bool wnan = __ballot(a1 != a1 || a2 != a2) != 0;
+      const double mw = wave_max_f64(fmax(a1, a2));
+      const unsigned long long b1 = __ballot(a1 == mw && a1 >= 0.0);
+      const unsigned long long b2 = __ballot(a2 == mw && a2 >= 0.0);
       if (lane == 0) {
-        red[w] = s1;
-        red[4 + w] = s2;
+        fred[w].best = mw;
+        fred[w].i1 = b1 ? w * 64 + __ffsll(b1) - 1 : -1;
+        fred[w].i2 = b2 ? 256 + w * 64 + __ffsll(b2) - 1 : -1;
+        fred[w].nan = wnan;
       }
       __syncthreads();
-      AmaxSum tot = red[0];
+      int pr = -1;
+      if (!(fred[0].nan | fred[1].nan | fred[2].nan | fred[3].nan)) {
+        // no NaN: the first row, in row order, holding the largest |a|
+        const double M = fmax(fmax(fred[0].best, fred[1].best), fmax(fred[2].best, fred[3].best));
 #pragma unroll
-      for (int q = 1; q < 8; ++q) tot = amax_combine(tot, red[q]);
-      const int pr = tot.bidx >= 0 ? tot.bidx : tot.nidx;
+        for (int q = 0; q < 4; ++q)
+          if (pr < 0 && fred[q].best == M && fred[q].i1 >= 0) pr = fred[q].i1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (pr < 0 && fred[q].best == M && fred[q].i2 >= 0) pr = fred[q].i2;
+      } else {
+        // idamax_'s NaN rule: the exact summaries, combined in row order
+        const AmaxSum none{0.0, -1, -1};
+        AmaxSum s1 = none, s2 = none;
+        if (t >= j && t < m) s1 = amax_of(a1, t);
+        if (t + 256 >= j && t + 256 < m) s2 = amax_of(a2, t + 256);
+        s1 = amax_wave(s1, lane);
+        s2 = amax_wave(s2, lane);
+        if (lane == 0) {
+          red[w] = s1;
+          red[4 + w] = s2;
+        }
+        __syncthreads();
+        AmaxSum tot = red[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) tot = amax_combine(tot, red[q]);
+        pr = tot.bidx >= 0 ? tot.bidx : tot.nidx;
+      }
       if (t == 0) {
         spiv[j] = pr - j;  // relative pivot
         allpiv[k + j] = pr - j;

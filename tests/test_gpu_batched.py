@@ -165,6 +165,21 @@ def test_batched_lu_vs_oracle(orc):
         assert res <= 10.0, (i, res)
 
 
+@pytest.mark.parametrize("col,rows", [(0, [7]), (20, [40, 300]), (20, [511]), (37, [0, 2, 500])])
+def test_batched_lu_nan_rule(orc, col, rows):
+    """A column holding NaNs takes idamax_'s rule (the scan restarts after
+    the last NaN): its pivot and those of the columns before it equal the
+    oracle's.  (The NaN then spreads through the trailing matrix.)"""
+    n = 512
+    A = np.asfortranarray(orc.gen_lu_c2(5, n, n))
+    for r in rows:
+        A[r, col] = np.nan
+    _, P, _, info = run_batch("lu", [A, orc.gen_lu_c2(6, 100, 100)])
+    _, pref, _ = orc.lu_piv(A)
+    assert np.array_equal(P[0][:col + 1], pref[:col + 1])
+    assert info[1] == fla.FLA_SUCCESS
+
+
 def test_batched_lu_null_pivot():
     A, L, U, p = exact_lu(100, 9, null_col=40)
     F, P, _, info = run_batch("lu", [A, exact_lu(60, 1)[0]])
