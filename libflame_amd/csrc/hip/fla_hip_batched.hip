@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -428,68 +429,84 @@ __device__ __forceinline__ void chol_panel_forward(const T* P, int kb, int m, T*
   }
 }
 
+// Column J of the register-resident Cholesky panel (compile-time J keeps
+// v in registers): the threads of rows < kb publish their column-J values,
+// then each thread scales its rows' column J and updates its rows' columns
+// J+1..min(r, kb-1).  jfail: the first column with a non-positive (or NaN)
+// diagonal; the later columns are skipped.
+template <typename T, int NB, int J>
+__device__ __forceinline__ void chol_reg_column(T (&v)[2][NB], T (*colb)[NB], int kb, int m, int t,
+                                                int& jfail) {
+  if (J < kb && jfail < 0) {
+    if (t < kb) colb[J & 1][t] = v[0][J];
+    __syncthreads();
+    const T d = colb[J & 1][J];
+    if (!(d > (T)0)) {
+      jfail = J;
+      return;
+    }
+    const T sd = sqrt(d);
+    const T inv = (T)1 / sd;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int r = t + 256 * q;
+      if (r == J) {
+        v[q][J] = sd;
+      } else if (r > J && r < m) {
+        const T lr = v[q][J] * inv;
+        v[q][J] = lr;
+#pragma unroll
+        for (int c = J + 1; c < NB; ++c)
+          if (c < kb && c <= r) v[q][c] -= lr * (colb[J & 1][c] * inv);
+      }
+    }
+  }
+}
+template <typename T, int NB, int... J>
+__device__ __forceinline__ void chol_reg_columns(T (&v)[2][NB], T (*colb)[NB], int kb, int m,
+                                                 int t, int& jfail,
+                                                 std::integer_sequence<int, J...>) {
+  (chol_reg_column<T, NB, J>(v, colb, kb, m, t, jfail), ...);
+}
+
 // ---- Cholesky of one matrix (lower), fused forward substitution -----------
 template <typename T, int NB>
 __device__ int chol_one(T* A, long lda, int n, T* B, long ldb, int nrhs, T* P, T* Ys, int t) {
+  __shared__ T colb[2][NB];  // the published column, by column parity
   int fail = -1;
   for (int k = 0; k < n; k += NB) {
     const int kb = min(NB, n - k), m = n - k;
-    // panel A[k:n, k:k+kb] -> P (ld MAXN)
-    load_panel<T, NB>(A + k + (long)k * lda, lda, kb, m, P, t);
-    __syncthreads();
-    // One barrier per column: thread t owns rows t and t + 256 of the panel.
-    // In the phase of column j every thread updates its rows' columns
-    // j+1..kb-1 with the scaled multipliers formed on the fly (the same
-    // rounded products l = a * (1/d) the stored column holds), and keeps its
-    // rows' scaled column-j values in registers; they are stored in the next
-    // phase, when no thread reads column j any more.  Operations and order
-    // per element are those of the two-barrier sweep (dpotf2's order).
-    T pend[2] = {(T)0, (T)0};
-    T psd = (T)0;
-    int pj = -1;  // the column whose scaled values are pending
-    for (int j = 0; j < kb; ++j) {
-      if (pj >= 0) {
+    // The panel A[k:n, k:k+kb] in registers: thread t owns rows t and
+    // t + 256.  Per column j, the threads of rows < kb publish their current
+    // column-j values (double-buffered by column parity, one barrier); every
+    // thread then scales its rows' column j (l = a * (1/d)) and updates its
+    // rows' columns j+1..min(r, kb-1) with l * (a_cj * (1/d)), the
+    // multipliers of the top rows formed from the published values.  These
+    // are dpotf2's operations in its order, per element.
+    T v[2][NB];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int r = t + 256 * q;
-          if (r > pj && r < m) P[r + pj * MAXN] = pend[q];
-        }
-        if (t == 0) P[pj + pj * MAXN] = psd;
-        pj = -1;
-      }
-      const T d = P[j + j * MAXN];
-      if (!(d > (T)0)) {
-        fail = k + j;
-        break;
-      }
-      const T sd = sqrt(d);
-      const T inv = (T)1 / sd;
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int c = 0; c < NB; ++c) {
         const int r = t + 256 * q;
-        if (r > j && r < m) {
-          const T lr = P[r + j * MAXN] * inv;
-          pend[q] = lr;
-          const int cmax = r < kb - 1 ? r : kb - 1;
-          for (int c = j + 1; c <= cmax; ++c) P[r + c * MAXN] -= lr * (P[c + j * MAXN] * inv);
+        v[q][c] = (r < m && c < kb) ? A[(k + r) + (long)(k + c) * lda] : (T)0;
+      }
+    int jfail = -1;
+    chol_reg_columns<T, NB>(v, colb, kb, m, t, jfail, std::make_integer_sequence<int, NB>{});
+    if (jfail >= 0) fail = k + jfail;
+    // the factored panel to LDS (trailing update, forward solve) and, lower
+    // part, back to A
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        const int r = t + 256 * q;
+        if (r < m && c < kb) {
+          P[r + c * MAXN] = v[q][c];
+          if (r >= c) A[(k + r) + (long)(k + c) * lda] = v[q][c];
         }
       }
-      psd = sd;
-      pj = j;
-      __syncthreads();
-    }
-    if (pj >= 0) {
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int r = t + 256 * q;
-        if (r > pj && r < m) P[r + pj * MAXN] = pend[q];
-      }
-      if (t == 0) P[pj + pj * MAXN] = psd;
-    }
     __syncthreads();
-    // write the factored panel back (lower part)
-    for (int c = 0; c < kb; ++c)
-      for (int r = t + c; r < m; r += 256) A[(k + r) + (long)(k + c) * lda] = P[r + c * MAXN];
     if (fail >= 0) return fail;
     // forward substitution with this panel: Y_k = L_kk^-1 B_k, B_rest -= L21 Y_k
     if (nrhs > 0) chol_panel_forward<T>(P, kb, m, B + k, ldb, nrhs, Ys, t);
@@ -513,6 +530,14 @@ __device__ void chol_back(const T* A, long lda, int n, T* B, long ldb, int nrhs,
   __syncthreads();
   for (int k = ((n - 1) / 16) * 16; k >= 0; k -= 16) {
     const int kb = min(16, n - k);
+    // the rows-above update's multipliers, loaded before the block solve
+    T l[2][16];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = t + 256 * q;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) l[q][j] = (i < k && j < kb) ? A[(k + j) + (long)i * lda] : (T)0;
+    }
     if (w == 0) {
       // row r of the block of L^T: L[k + j, k + r]
       T cf[16];
@@ -534,17 +559,18 @@ __device__ void chol_back(const T* A, long lda, int n, T* B, long ldb, int nrhs,
       }
     }
     __syncthreads();
-    for (int i = t; i < k; i += 256) {
-      T l[16];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) l[j] = j < kb ? A[(k + j) + (long)i * lda] : (T)0;
+    for (int q = 0; q < 2; ++q) {
+      const int i = t + 256 * q;
+      if (i < k) {
 #pragma unroll 1
-      for (int c = 0; c < nrhs; ++c) {
-        T v = X[i + c * MAXN];
+        for (int c = 0; c < nrhs; ++c) {
+          T v = X[i + c * MAXN];
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
-          if (j < kb) v -= l[j] * X[(k + j) + c * MAXN];
-        X[i + c * MAXN] = v;
+          for (int j = 0; j < 16; ++j)
+            if (j < kb) v -= l[q][j] * X[(k + j) + c * MAXN];
+          X[i + c * MAXN] = v;
+        }
       }
     }
     __syncthreads();
@@ -798,6 +824,14 @@ __device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T
   __syncthreads();
   for (int k = ((n - 1) / 16) * 16; k >= 0; k -= 16) {
     const int kb = min(16, n - k);
+    // the rows-above update's coefficients, loaded before the block solve
+    T u[2][16];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = t + 256 * q;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u[q][j] = (i < k && j < kb) ? A[i + (long)(k + j) * lda] : (T)0;
+    }
     if (w == 0) {
       T cf[16];
 #pragma unroll
@@ -818,17 +852,18 @@ __device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T
       }
     }
     __syncthreads();
-    for (int i = t; i < k; i += 256) {
-      T u[16];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) u[j] = j < kb ? A[i + (long)(k + j) * lda] : (T)0;
+    for (int q = 0; q < 2; ++q) {
+      const int i = t + 256 * q;
+      if (i < k) {
 #pragma unroll 1
-      for (int c = 0; c < nrhs; ++c) {
-        T v = X[i + c * MAXN];
+        for (int c = 0; c < nrhs; ++c) {
+          T v = X[i + c * MAXN];
 #pragma unroll
-        for (int j = 15; j >= 0; --j)
-          if (j < kb) v -= u[j] * X[(k + j) + c * MAXN];
-        X[i + c * MAXN] = v;
+          for (int j = 15; j >= 0; --j)
+            if (j < kb) v -= u[q][j] * X[(k + j) + c * MAXN];
+          X[i + c * MAXN] = v;
+        }
       }
     }
     __syncthreads();
