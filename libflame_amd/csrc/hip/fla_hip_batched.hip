@@ -50,6 +50,19 @@ struct BatchItem {
   int pad;
 };
 
+// A matrix pointer read from the item array as a global-address-space
+// value: the accesses derived from it compile to global loads and stores.
+// Read as a plain pointer it is generic, and flat accesses share the LDS
+// counter, so every LDS read waits for the outstanding memory loads (the
+// LU's interchanges, which read their slot lists from LDS between loads,
+// ran twice as long that way).
+typedef __attribute__((address_space(1))) void* global_void_ptr;
+template <typename T>
+__device__ __forceinline__ T* global_field(void* const* field) {
+  const global_void_ptr g = *reinterpret_cast<const global_void_ptr*>(field);
+  return (T*)g;
+}
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -818,7 +831,15 @@ __device__ int lu_one(T* A, long lda, int n, int* ipiv, T* B, long ldb, int nrhs
 // updates the rows above, one term at a time in descending column order.
 // Per element this is dtrsm_'s column sweep for B := inv(U) B.
 template <typename T>
-__device__ void lu_back(const T* A, long lda, int n, T* B, long ldb, int nrhs, T* X, int t) {
+__device__ void lu_back(const BatchItem* itp, int c0, int nrhs,
+                        __attribute__((address_space(3))) T* X, int t) {
+  // out of line (inlined, it pushes the LU kernel past its register
+  // budget), so it reads its pointers from the item as global values and
+  // takes X as an LDS pointer
+  const T* A = global_field<T>(&itp->A);
+  T* B = global_field<T>(&itp->B) + (long)c0 * itp->ldb;
+  const long lda = itp->lda, ldb = itp->ldb;
+  const int n = itp->n;
   const int w = t >> 6, lane = t & 63, r = lane & 15, g = lane >> 4;
   for (int e = t; e < n * nrhs; e += 256) X[e % n + (e / n) * MAXN] = B[e % n + (long)(e / n) * ldb];
   __syncthreads();
@@ -896,8 +917,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     if (idx >= count) break;
     const BatchItem it = items[idx];
-    T* A = static_cast<T*>(it.A);
-    T* B = static_cast<T*>(it.B);
+    // global pointers for the LU; the Cholesky kernels measured faster with
+    // the generic ones
+    T* A = OP == 1 ? global_field<T>(&items[idx].A) : static_cast<T*>(it.A);
+    T* B = OP == 1 ? global_field<T>(&items[idx].B) : static_cast<T*>(it.B);
     const int nr = it.B ? it.nrhs : 0;
     int r;
     if (OP == 0) {
@@ -908,11 +931,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
           chol_back<T>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), P, t);
       }
     } else {
-      r = lu_one<T, NB>(A, it.lda, it.n, it.ipiv, B, it.ldb, nr, P, red, spiv, Ys, t);
+      r = lu_one<T, NB>(A, it.lda, it.n, global_field<int>((void* const*)&items[idx].ipiv), B,
+                        it.ldb, nr, P, red, spiv, Ys, t);
       __syncthreads();
       if (r < 0 && nr > 0) {
         for (int c0 = 0; c0 < nr; c0 += 8)
-          lu_back<T>(A, it.lda, it.n, B + (long)c0 * it.ldb, it.ldb, min(8, nr - c0), P, t);
+          lu_back<T>(&items[idx], c0, min(8, nr - c0), (__attribute__((address_space(3))) T*)P, t);
       }
     }
     if (t == 0) info[it.index] = r < 0 ? kSuccess : r;
