@@ -205,7 +205,7 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
     return C + (ti * 32 + li) + ((long)tj * 32 + Acc<T>::dj(lk, 0)) * ldc;
   };
   // loads C and (PC_GLOBAL) the column fragments of one tile
-  auto load = [&](int ti, int tj, T (&cv)[2][2][4], T (&pf)[NK][2]) {
+  auto load = [&](int ti, int tj, T (&cv)[2][2][4], T (&pf)[NK][2], bool frags) {
     if (interior(ti, tj)) {
       const T* cp = cbase(ti, tj);
 #pragma unroll
@@ -225,7 +225,7 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
             cv[a][b][r] = in_c(i, j) ? C[i + (long)j * ldc] : (T)0;
           }
     }
-    if (PC_GLOBAL) {
+    if (PC_GLOBAL && frags) {
       if (kb == NB && (tj + 1) * 32 <= n2) {
         const T* pp = Pc + (long)(tj * 32 + li) * pcs + (long)lk * pks;
 #pragma unroll
@@ -250,7 +250,7 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
   int ti, tj;
   coords(tile, ti, tj);
   T cn[2][2][4], pn[NK][2];
-  load(ti, tj, cn, pn);
+  load(ti, tj, cn, pn, true);
   for (; tile < ntiles; tile += 4) {
     T cv[2][2][4], pf[NK][2];
 #pragma unroll
@@ -267,7 +267,9 @@ __device__ void tile_update(int m2, int n2, int kb, const T* Pr, const T* Pc, lo
     const int ci = ti, cj = tj;
     if (tile + 4 < ntiles) {
       coords(tile + 4, ti, tj);
-      load(ti, tj, cn, pn);
+      // a wave's next tile is mostly in the same tile column: its column
+      // fragments (from memory) are then the ones already held
+      load(ti, tj, cn, pn, tj != cj);
     }
     AT acc[2][2];
 #pragma unroll
