@@ -327,10 +327,11 @@ def _sym_inf_norm(fla, np, A0, n):
     return best
 
 
-def bench_batched(fla, np, count, seed=2026):
+def bench_batched(fla, np, count, seed=2026, tune=True):
     """4096 SPD matrices, n_i ~ U{32..512}, A_i = B_i B_i^T/n_i + 0.1 I (FP64 and
     FP32), and 4096 U[-1,1] matrices for LU; each with 1-8 N(0,1) RHS; one
-    persistent launch each (autotuned configuration, graph replay)."""
+    persistent launch each (autotuned configuration unless tune=False, graph
+    replay)."""
     import ctypes as C
     lib = fla.lib()
     rng = np.random.default_rng(seed)
@@ -397,14 +398,16 @@ def bench_batched(fla, np, count, seed=2026):
             fla.check(lib.fla_hip_gen_uniform(seed + 7, tot, 1, A0.ptr, tot))
             fla.check(lib.fla_hip_gen_normal(seed + 99991, totb, 1, B0.ptr, totb))
         nb, wg, tms = C.c_int(), C.c_int(), C.c_double()
-        fla.check(lib.fla_hip_batched_tune(kind, count, ni, ri, C.byref(nb), C.byref(wg),
-                                           C.byref(tms)))
+        if tune:
+            fla.check(lib.fla_hip_batched_tune(kind, count, ni, ri, C.byref(nb), C.byref(wg),
+                                               C.byref(tms)))
         fn = {0: lib.fla_hip_dposv_batched, 1: lib.fla_hip_sposv_batched,
               2: lib.fla_hip_dgesv_batched}[kind]
         t, bad = run(fn, ptrs(A.addr, offs, esz), ptrs(Bw.addr, offb, esz), esz, lu=(kind == 2))
         fl = flops_lu if kind == 2 else flops_chol
         out[name] = {"ms": round(t * 1e3, 3), "tflops": round(fl / t / 1e12, 3),
-                     "failed": bad, "tuned_nb": nb.value, "tuned_wg_per_cu": wg.value}
+                     "failed": bad, "tuned_nb": nb.value if tune else None,
+                     "tuned_wg_per_cu": wg.value if tune else None}
         log(f"batched {name}: {out[name]}")
     del A0, A, B0, Bw, info, piv, Bg
     return out
@@ -622,6 +625,10 @@ def main():
     ap.add_argument("--adj-n", type=int, default=8192)
     ap.add_argument("--no-inv", action="store_true")
     ap.add_argument("--no-flash", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched leg, untuned (default configuration), and "
+                         "print its results; for counter passes, where the tuner's many small "
+                         "launches are slow")
     ap.add_argument("--inv-n", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -633,6 +640,11 @@ def main():
 
     import libflame_amd as fla
     fla.FLA_Init()
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(fla, np, args.batch_count, tune=False)}),
+              flush=True)
+        fla.FLA_Finalize()
+        return
     if args.chol_nb:
         fla.set_blocksizes(chol_nb=args.chol_nb)
     n = args.n
