@@ -251,18 +251,44 @@ int launch_potrf_diag_inv(hipStream_t s, int nb, double* A, long lda, int col_of
 // of 32 held in VGPRs: first the contributions of the already solved columns
 // k (ascending, re-read by the thread that wrote them), then the 32 x 32
 // diagonal block by substitution -- dtrsm_('R','L','T','N')'s update order
-// (src/flablas/f2c/dtrsm.c), temp = 1/a_kk multiplied in.  L is read with
-// wave-uniform (scalar) loads.
+// (src/flablas/f2c/dtrsm.c), temp = 1/a_kk multiplied in.  L's columns
+// below the diagonal blocks are read with wave-uniform (scalar) loads, 32
+// at a time; the four 32 x 32 diagonal blocks, whose substitution would
+// otherwise wait on dozens of dependent scalar loads each, are first copied
+// into LDS (16.5 KB packed, small enough to share a CU with the update
+// kernel).
 template <bool FULL>
 __global__ __launch_bounds__(256) void trsm_rlt_rows_kernel(int m, int nb,
                                                             const double* __restrict__ L,
                                                             long ldl, double* __restrict__ X,
                                                             long ldx, const int* __restrict__ abort) {
+  // diagonal block b, lower triangle packed by columns: (r, c), r >= c, at
+  // Ld[b][c*LB - c*(c-1)/2 + (r - c)]
+  __shared__ double Ld[PB / LB][LB * (LB + 1) / 2];
   if (abort && *abort >= 0) return;
+  const int nbk = FULL ? PB : nb;
+  {
+    // 16 elements per thread, all loads issued before the LDS stores
+    double v[(PB / LB) * LB * LB / 256];
+#pragma unroll
+    for (int q = 0; q < (PB / LB) * LB * LB / 256; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      const int b = e / (LB * LB), r = e % LB, c = (e / LB) % LB;
+      const int gr = b * LB + r, gc = b * LB + c;
+      v[q] = (gr < nbk && gc < nbk && r >= c) ? L[gr + (long)gc * ldl] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < (PB / LB) * LB * LB / 256; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      const int b = e / (LB * LB), r = e % LB, c = (e / LB) % LB;
+      if (b * LB + r < nbk && b * LB + c < nbk && r >= c)
+        Ld[b][c * LB - (c * (c - 1)) / 2 + (r - c)] = v[q];
+    }
+  }
+  __syncthreads();
   const int row = blockIdx.x * 256 + threadIdx.x;
   if (row >= m) return;
   double* xr = X + row;
-  const int nbk = FULL ? PB : nb;
   for (int jb = 0; jb < nbk; jb += LB) {
     const int w = FULL ? LB : ((nbk - jb) < LB ? (nbk - jb) : LB);
     double x[LB];
@@ -275,13 +301,15 @@ __global__ __launch_bounds__(256) void trsm_rlt_rows_kernel(int m, int nb,
       for (int c = 0; c < LB; ++c)
         if (FULL || c < w) x[c] -= lk[c] * xk;
     }
+    const double* Lb = Ld[jb / LB];
 #pragma unroll
     for (int j = 0; j < LB; ++j) {
       if (FULL || j < w) {
-        x[j] *= 1.0 / L[(long)(jb + j) + (long)(jb + j) * ldl];
+        const double* lj = Lb + (j * LB - (j * (j - 1)) / 2);  // column j from its diagonal
+        x[j] *= 1.0 / lj[0];
 #pragma unroll
         for (int c = j + 1; c < LB; ++c)
-          if (FULL || c < w) x[c] -= L[(long)(jb + c) + (long)(jb + j) * ldl] * x[j];
+          if (FULL || c < w) x[c] -= lj[c - j] * x[j];
       }
     }
 #pragma unroll
