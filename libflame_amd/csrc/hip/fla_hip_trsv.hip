@@ -128,16 +128,73 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
   const int ndep = FWD ? I : nb - 1 - I;
   auto dep_col = [&](int s) { return (FWD ? s : nb - 1 - s) * NB; };
   auto full_tile = [&](int c0) { return (r0 + NB <= n) && (c0 + NB <= n); };
-  double fa[32], fb[32];
-  if (comp && ndep > 0) load_frag<TRANS>(fa, T, ldt, n, row, dep_col(0), lk, full_tile(dep_col(0)));
+  double fa[32];  // op(T)_IJ fragments of the next dependency
 
-  // One dependency: prefetch the next fragments, wait for X_J, stage it, fold it in.
-  auto step = [&](int s, double (&cur)[32], double (&nxt)[32]) -> bool {
-    const int c0 = dep_col(s);
-    if (comp && s + 1 < ndep) {
-      const int c1 = dep_col(s + 1);
-      load_frag<TRANS>(nxt, T, ldt, n, row, c1, lk, full_tile(c1));
+  // Inverse path with dependencies: the last dependency J (the block solved
+  // just before this one) is folded through V = op(W_I) op(T)_IJ, formed
+  // here at launch, so that after X_J arrives one 128 x 128 x 16 product
+  // remains instead of two (X_I = op(W_I) Y_I - V X_J, where Y_I has every
+  // other dependency folded in).  V(row, k) for this lane's row sits in
+  // vf[4t + r] with k = 16t + 4r + lk: the MFMA result layout of V^T's
+  // 16-column tile t, used directly as the B operand.  The fragment buffer
+  // is single: vf takes the registers of a second one.
+  const bool pre = !subst && ndep > 0;
+  const int nfold = pre ? ndep - 1 : ndep;  // dependencies folded into Y_I
+  double vf[32];
+  if (pre) {
+    // Tile tt of op(T)_IJ (rows r0.., columns cl + 16 tt ..) goes through Xf
+    // in the X_J staging order: the loader waves fetch tile tt + 1 while the
+    // compute waves multiply tile tt.
+    const int cl = dep_col(ndep - 1);
+    double tv[8];
+    auto fetch = [&](int tt) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = lt + 256 * q;
+        const int m = TRANS ? (e >> 4) : (e & (NB - 1)), i = TRANS ? (e & 15) : (e >> 7);
+        const int r = r0 + m, c = cl + 16 * tt + i;
+        tv[q] = (r < n && c < n) ? (TRANS ? T[(long)c + (long)r * ldt] : T[(long)r + (long)c * ldt])
+                                 : 0.0;
+      }
+    };
+    auto put = [&]() {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = lt + 256 * q;
+        const int m = TRANS ? (e >> 4) : (e & (NB - 1)), i = TRANS ? (e & 15) : (e >> 7);
+        Xf[(m & 31) * 64 + (m >> 5) * 16 + i] = tv[q];
+      }
+    };
+    if (!comp) {
+      fetch(0);
+      put();
     }
+    __syncthreads();  // Wf and tile 0 staged
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) {
+      if (!comp && tt + 1 < 8) fetch(tt + 1);
+      if (comp) {
+        // V^T tile: sum over m of op(T)(r0 + m, cl + 16 tt + col) op(W_I)(16 w + row, m)
+        d4 v = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks)
+          v = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane],
+                                                   Wf[(w * 32 + ks) * 64 + lane], v, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) vf[4 * tt + r] = v[r];
+      }
+      __syncthreads();
+      if (tt + 1 < 8) {
+        if (!comp) put();
+        __syncthreads();
+      }
+    }
+  }
+  if (comp && nfold > 0)
+    load_frag<TRANS>(fa, T, ldt, n, row, dep_col(0), lk, full_tile(dep_col(0)));
+
+  // Wait until block s of the processing order is final; false when the wait failed.
+  auto wait_final = [&](int s) -> bool {
     if (lt == 0) {
       const unsigned long long t0 = wall_clock64();
       while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= s) {
@@ -151,7 +208,12 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
       }
     }
     __syncthreads();
-    if (bail) return false;
+    return bail == 0;
+  };
+  // Loader waves: -X_J -> Xf, its row k placed as the contraction index of
+  // MFMA step ks, lane group lk: k = 32 lk + ks against op(T) fragments,
+  // k = 16 (ks / 4) + 4 (ks % 4) + lk against vf.
+  auto stage_dep = [&](int c0, bool vlayout) {
     if (!comp) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // Xm is uncached
       for (int e = lt; e < NB * RHS; e += 256) {
@@ -160,25 +222,54 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
                              ? __hip_atomic_load(&Xm[(long)(c0 + k) + (long)col * n],
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                              : 0.0;
-        Xf[(k & 31) * 64 + (k >> 5) * 16 + col] = -v;
+        const int slot = vlayout ? (4 * (k >> 4) + ((k >> 2) & 3)) * 64 + (k & 3) * 16
+                                 : (k & 31) * 64 + (k >> 5) * 16;
+        Xf[slot + col] = -v;
       }
+    }
+    __syncthreads();
+  };
+
+  // One dependency: wait for X_J, stage it, fold it in, then start loading
+  // the next dependency's fragments (they arrive while the next wait runs).
+  for (int s = 0; s < nfold; ++s) {
+    if (!wait_final(s)) return;
+    stage_dep(dep_col(s), false);
+    if (comp) {
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], fa[ks], acc, 0, 0, 0);
+      if (s + 1 < nfold) {
+        const int c1 = dep_col(s + 1);
+        load_frag<TRANS>(fa, T, ldt, n, row, c1, lk, full_tile(c1));
+      }
+    }
+    __syncthreads();
+  }
+
+  d4 out = d4{0.0, 0.0, 0.0, 0.0};
+  if (pre) {
+    // out = op(W_I) Y_I while the loader polls for X_J, then out -= V X_J.
+    if (comp) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Xf[(16 * (w & 1) + li) * 64 + (w >> 1) * 16 + lk + 4 * r] = acc[r];
     }
     __syncthreads();
     if (comp) {
 #pragma unroll
       for (int ks = 0; ks < 32; ++ks)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], cur[ks], acc, 0, 0, 0);
+        out = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane],
+                                                   Wf[(w * 32 + ks) * 64 + lane], out, 0, 0, 0);
     }
-    __syncthreads();
-    return true;
-  };
-  for (int s = 0; s < ndep; s += 2) {
-    if (!step(s, fa, fb)) return;
-    if (s + 1 < ndep && !step(s + 1, fb, fa)) return;
-  }
-
-  d4 out = d4{0.0, 0.0, 0.0, 0.0};
-  if (!subst) {
+    if (!wait_final(ndep - 1)) return;  // its barrier also ends the reads of Xf
+    stage_dep(dep_col(ndep - 1), true);
+    if (comp) {
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks)
+        out = __builtin_amdgcn_mfma_f64_16x16x4f64(Xf[ks * 64 + lane], vf[ks], out, 0, 0, 0);
+    }
+  } else if (!subst) {
     // Y_I -> Xf as the contraction operand (its row k = 16w + li is the
     // contraction index), then X_I = op(W_I) Y_I.
     if (comp) {
