@@ -94,11 +94,24 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
 
   if (!subst) {
     // op(W_I)(row, k) -> Wf[((row/16)*32 + k%32)*64 + (k/32)*16 + row%16]
+    // eight loads in flight per thread before the LDS writes
     const double* Wi = W + (long)I * NB * NB;
-    for (int e = t; e < NB * NB; e += NTHREADS) {
-      const int p = e & (NB - 1), q = e >> 7;  // stored W(p, q)
-      const int row = TRANS ? q : p, k = TRANS ? p : q;
-      Wf[((row >> 4) * 32 + (k & 31)) * 64 + (k >> 5) * 16 + (row & 15)] = Wi[e];
+    for (int e0 = t; e0 < NB * NB; e0 += 8 * NTHREADS) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * NTHREADS;
+        v[u] = e < NB * NB ? Wi[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * NTHREADS;
+        if (e < NB * NB) {
+          const int p = e & (NB - 1), q = e >> 7;  // stored W(p, q)
+          const int row = TRANS ? q : p, k = TRANS ? p : q;
+          Wf[((row >> 4) * 32 + (k & 31)) * 64 + (k >> 5) * 16 + (row & 15)] = v[u];
+        }
+      }
     }
   } else {
     // M(i, c) = op(T)(r0 + i, r0 + c) = TRANS ? T(r0 + c, r0 + i) : T(r0 + i, r0 + c)
@@ -213,19 +226,28 @@ __global__ __launch_bounds__(NTHREADS) void trsm_left_narrow_kernel(int n, int n
   // Loader waves: -X_J -> Xf, its row k placed as the contraction index of
   // MFMA step ks, lane group lk: k = 32 lk + ks against op(T) fragments,
   // k = 16 (ks / 4) + 4 (ks % 4) + lk against vf.
+  // Thread lt stages row k = lt % 128 of columns lt / 128 + 2q, q = 0..7:
+  // the eight loads are all issued before the first LDS write, so their
+  // round trips to the uncached copy overlap.
   auto stage_dep = [&](int c0, bool vlayout) {
     if (!comp) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // Xm is uncached
-      for (int e = lt; e < NB * RHS; e += 256) {
-        const int k = e & (NB - 1), col = e >> 7;
-        const double v = (c0 + k < n && col < nrhs)
-                             ? __hip_atomic_load(&Xm[(long)(c0 + k) + (long)col * n],
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                             : 0.0;
-        const int slot = vlayout ? (4 * (k >> 4) + ((k >> 2) & 3)) * 64 + (k & 3) * 16
-                                 : (k & 31) * 64 + (k >> 5) * 16;
-        Xf[slot + col] = -v;
+      const int k = lt & (NB - 1), col0 = lt >> 7;
+      const bool rin = c0 + k < n;
+      // one per-lane base; the column steps 2 q n are uniform
+      double* const xb = Xm + (long)(c0 + k) + (long)col0 * n;
+      double v[NB * RHS / 256];
+#pragma unroll
+      for (int q = 0; q < NB * RHS / 256; ++q) {
+        v[q] = (rin && col0 + 2 * q < nrhs)
+                   ? __hip_atomic_load(xb + (long)(2 * q) * n, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)
+                   : 0.0;
       }
+      const int slot = vlayout ? (4 * (k >> 4) + ((k >> 2) & 3)) * 64 + (k & 3) * 16
+                               : (k & 31) * 64 + (k >> 5) * 16;
+#pragma unroll
+      for (int q = 0; q < NB * RHS / 256; ++q) Xf[slot + col0 + 2 * q] = -v[q];
     }
     __syncthreads();
   };
